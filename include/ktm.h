@@ -1,0 +1,247 @@
+/*
+ * ktm.h -- C ABI of the MI355X Kirchhoff prestack time-migration engine.
+ *
+ * The reference program (obriengareth/Random, Kirchhoff.c) exposes one entry
+ * point, `void kirchhoff(int job_id)` (Kirchhoff.c:92, 1104), fed through
+ * process globals by main()'s setup (Kirchhoff.c:215-373).  This header keeps
+ * that signature as a compatibility shim and adds explicit-argument entry
+ * points.  Every function returns a ktm_status (no exit(), unlike the
+ * reference's error paths at Kirchhoff.c:1122-1136,1415-1420).
+ *
+ * Pointers documented as "device" are HIP device memory; `stream` is a
+ * hipStream_t passed as void* (NULL = the null stream).  A plan is not
+ * thread-safe; distinct plans may be used from distinct threads.
+ */
+#ifndef KTM_H
+#define KTM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ktm_status {
+    KTM_OK = 0,
+    KTM_ERR_ARG = -1,          /* bad argument or inconsistent configuration   */
+    KTM_ERR_IO = -2,           /* fopen/fread/fwrite failure (Kirchhoff.c:1122) */
+    KTM_ERR_FORMAT = -3,       /* SEG-Y file shorter than its headers claim     */
+    KTM_ERR_VELOCITY = -4,     /* velocity <= 0, or a grid index below zero     */
+    KTM_ERR_NOMEM = -5,        /* host or device allocation failed              */
+    KTM_ERR_DEVICE = -6,       /* HIP runtime error or no device                */
+    KTM_ERR_UNSUPPORTED = -7   /* mode/size outside what the kernels implement  */
+} ktm_status;
+
+/* Configuration: the compile-time macros of Kirchhoff.c:17-65 as run-time
+ * fields (same names, same types: integer macros are int32, decimal ones
+ * double), plus the three geometry variants of Kirchhoff_GHTest1.c. */
+typedef struct ktm_config {
+    int32_t start_iline, end_iline;      /* Startiline, Endiline   Kirchhoff.c:34-35 */
+    int32_t start_xline, end_xline;      /* Startxline, Endxline   Kirchhoff.c:36-37 */
+    double  inc_iline, inc_xline;        /* INCiline, INCxline     Kirchhoff.c:40-41 */
+    double  dh;                          /* dh                     Kirchhoff.c:44    */
+    int32_t nh;                          /* nh                     Kirchhoff.c:45    */
+    int32_t reserved0;
+    double  oh;                          /* oh                     Kirchhoff.c:46    */
+    double  ox, oy, angle_e_to_n;        /* ox, oy, angleEtoN      Kirchhoff.c:49-51 */
+    double  iline_distance;              /* iline_distance         Kirchhoff.c:54    */
+    double  xline_distance;              /* xline_distance         Kirchhoff.c:55    */
+    double  aperture;                    /* aperture               Kirchhoff.c:58    */
+    double  anti_alias_cutoff;           /* anti_alias_cutoff      Kirchhoff.c:59    */
+    int32_t nvel_ilines, nvel_xlines;    /* NVel_ilines/xlines     Kirchhoff.c:22-23 */
+    int32_t dvel_ilines, dvel_xlines;    /* dVel_ilines/xlines     Kirchhoff.c:24-25 */
+    int32_t ovel_ilines, ovel_xlines;    /* oVel_ilines/xlines     Kirchhoff.c:26-27 */
+    int32_t ntblock;                     /* ntblock                Kirchhoff.c:63    */
+    int32_t jobs_x, jobs_y;              /* number_mpi_jobsX/Y     Kirchhoff.c:64-65 */
+    int32_t image_origin_zero;           /* Kirchhoff_GHTest1.c:1346-1348,1365-1366  */
+    int32_t vel_lower_clamp;             /* Kirchhoff_GHTest1.c:1395-1399            */
+    int32_t reserved1;
+    double  scale_xy;                    /* 0 = Kirchhoff.c; >0 = scaleXY of
+                                            Kirchhoff_GHTest1.c:56,1302-1315        */
+} ktm_config;
+
+/* The reference's own macro values (Kirchhoff.c:17-65). */
+void ktm_config_default(ktm_config *cfg);
+
+/* What main() derives before the job loop: count_size_file (ns, dt, nt,
+ * Kirchhoff.c:633-722), create_aliasfilter (trangeMax, Kirchhoff.c:380-441),
+ * the slab decomposition (Kirchhoff.c:266-300) and the survey rotation
+ * (Kirchhoff.c:312-313).  nxp/nyp = nx*jobs_x, ny*jobs_y: the padded image
+ * that the jobs cover together (the last slab is not clipped at Endiline). */
+typedef struct ktm_setup {
+    int32_t ns, nsm;
+    float   dt, dtm;
+    int32_t trange_max;
+    int32_t Nx, Ny, nx, ny, no_job;
+    int32_t nxp, nyp, nh;
+    float   sinA, cosA;
+    int32_t reserved;
+} ktm_setup;
+
+/* Setup without a device: fills *out; anti_alias (may be NULL) gets ns floats.
+ * Replaces create_aliasfilter() Kirchhoff.c:380-441 and main() 266-313. */
+int ktm_setup_init(const ktm_config *cfg, int ns, float dt, int nsm, float dtm,
+                   ktm_setup *out, float *anti_alias);
+
+/* Per-trace geometry and scale as the kernels consume it (32 bytes).
+ * sxt..cmpy: rotated source, receiver and CMP (Kirchhoff.c:1296-1309);
+ * off: offset class (Kirchhoff.c:1310); scale: per-trace FP32 scale of a
+ * BF16/FP16-stored trace (1 for FP32 storage). */
+typedef struct ktm_trace_geom {
+    float   sxt, syt, gxt, gyt, cmpx, cmpy;
+    int32_t off;
+    float   scale;
+} ktm_trace_geom;
+
+/* How a trace is turned into the filtered trace F(T) that migration sums.
+ * REFERENCE: the FP32 double integral of Kirchhoff.c:1312-1328 followed by
+ *   the trange=2 second difference of Kirchhoff.c:1391-1392, bit for bit.
+ * CLOSED: the same linear operator evaluated as the 3-tap [1,2,1] filter
+ *   F(T) = x[T] + 2x[T+1] + x[T+2] on x[t] = tr[t]/(4 t2[t]) (with the
+ *   mirror-pad and tail cases of T < 2 and T >= ns-2), which avoids the
+ *   catastrophic cancellation of the FP32 prefix sums. */
+enum { KTM_PREP_REFERENCE = 0, KTM_PREP_CLOSED = 1 };
+/* Storage type of the traces the migration kernel stages. */
+enum { KTM_TRACE_F32 = 0, KTM_TRACE_BF16 = 1, KTM_TRACE_F16 = 2 };
+/* Image accumulation: plain FP32 adds in trace order (Kirchhoff.c:1394),
+ * or FP32 with a Kahan compensation image. */
+enum { KTM_ACCUM_PLAIN = 0, KTM_ACCUM_KAHAN = 1 };
+
+/* Device-resident setup: velocity model (squared, Kirchhoff.c:617), traveltime
+ * tables and the image-column tables of one configuration. */
+typedef struct ktm_plan ktm_plan;
+
+/* vel: host array [nvel_ilines][nvel_xlines][nsm] of RMS velocities as read
+ * from the velocity SEG-Y (Kirchhoff.c:545).  Fails with KTM_ERR_VELOCITY if
+ * an image column maps below the model (a read out of bounds in the
+ * reference, Kirchhoff.c:1367-1372) or a velocity used by the image is <= 0.
+ * Replaces build_vel_model() Kirchhoff.c:449-626 plus the per-column index
+ * arithmetic of Kirchhoff.c:1346-1365,1376-1378. */
+int  ktm_plan_create(ktm_plan **plan, const ktm_config *cfg, int ns, float dt,
+                     int nsm, float dtm, const float *vel);
+void ktm_plan_destroy(ktm_plan *plan);
+int  ktm_plan_setup(const ktm_plan *plan, ktm_setup *out);
+/* Row length (elements) of one prepared trace in the buffers below. */
+int64_t ktm_plan_trace_row(const ktm_plan *plan);
+/* Elements of one image array: nxp*nyp*nh*ns. */
+int64_t ktm_plan_image_size(const ktm_plan *plan);
+
+/* Trace preparation (device): traces[ntr][ns] IEEE samples, coords[ntr][4]
+ * header words sx,sy,gx,gy (bytes 73-88, host order, >= 0).  Writes geom[ntr]
+ * and, for F32 storage, prepared[ntr][trace_row] floats holding F(T) for
+ * T < ns (zero beyond); for BF16/F16 storage, prepared holds the raw trace
+ * quantised with a per-trace max-abs scale (geom.scale) and the filter runs
+ * inside the migration kernel.  Replaces Kirchhoff.c:1296-1328. */
+int ktm_prep(const ktm_plan *plan, const float *traces, const int32_t *coords,
+             int64_t ntr, int prep_mode, int trace_dtype, void *prepared,
+             ktm_trace_geom *geom, void *stream);
+
+/* km = 0, fold = 1, comp = 0 (Kirchhoff.c:1262-1274); comp may be NULL. */
+int ktm_image_reset(const ktm_plan *plan, float *km, float *fold, float *comp,
+                    void *stream);
+
+/* Migrate one block of prepared traces into the resident image (device):
+ * the trace loop of Kirchhoff.c:1292-1406 for every job slab at once.
+ * km/fold/comp are [nxp][nyp][nh][ns]; comp is needed for KAHAN only.
+ * Accumulation continues from the current contents, in trace order, so
+ * successive blocks equal one pass over their concatenation.  If counter is
+ * non-NULL (device uint64) the number of contributions (trace, column) x ns
+ * is added to it. */
+int ktm_migrate_block(const ktm_plan *plan, const void *prepared, int trace_dtype,
+                      int prep_mode, const ktm_trace_geom *geom, int64_t ntr,
+                      int accum, int job, float *km, float *fold, float *comp,
+                      uint64_t *counter, void *stream);
+/* job < 0 migrates every slab; job >= 0 only the columns of that job's slab
+ * (Kirchhoff.c:1341-1347), leaving the rest of the image untouched. */
+
+/* Geometry only (device): coords[ntr][4] -> geom[ntr], Kirchhoff.c:1296-1310. */
+int ktm_geometry(const ktm_plan *plan, const int32_t *coords, int64_t ntr,
+                 ktm_trace_geom *geom, void *stream);
+
+/* Device memory and stream helpers for C callers (thin wrappers over HIP). */
+int ktm_device_alloc(void **ptr, size_t bytes);
+int ktm_device_free(void *ptr);
+int ktm_copy_to_device(void *dst, const void *src, size_t bytes);
+int ktm_copy_to_host(void *dst, const void *src, size_t bytes);
+int ktm_device_sync(void);
+
+/* Post-stack (device): out = low-pass(-dt*t*dkm/dt*dt/fold) per image trace,
+ * Kirchhoff.c:1423-1479, as an exact circular convolution with the
+ * real kernel whose DFT is anti_alias (FP64 arithmetic). */
+int ktm_poststack(const ktm_plan *plan, const float *km, const float *fold,
+                  float *out, void *stream);
+
+
+/* Host convenience: whole reference job loop for all jobs (Kirchhoff.c:
+ * 347-350 on one rank): prep + migrate in blocks of cfg->ntblock + post-stack.
+ * Host arrays; km/fold/image are [nxp][nyp][nh][ns] (any may be NULL). */
+typedef struct ktm_stats {
+    uint64_t contributions;   /* (trace, column) pairs in aperture x ns */
+    double   migrate_ms;      /* device time of the migration kernels   */
+    double   total_ms;        /* host wall time of the call             */
+} ktm_stats;
+
+int ktm_migrate(const ktm_config *cfg, int ns, float dt, const float *traces,
+                const int32_t *coords, int64_t ntr, const float *vel, int nsm,
+                float dtm, float *km, float *fold, float *image, ktm_stats *stats);
+
+/* ---- SEG-Y (host), READ_SEGY.c:29-115,169-450 and Kirchhoff.c:633-874 ---- */
+
+/* IBM hex float words in file byte order -> IEEE (Kirchhoff.c:882-895,
+ * 1284-1290, including its overflow-to-inf and 0*inf=NaN cases). */
+void ktm_ibm_to_float(const uint8_t *bytes, int64_t nwords, float *out);
+/* count_size_file (Kirchhoff.c:633-722): ns, dt from trace 1, nt from size. */
+int  ktm_segy_count(const char *path, int *ns, float *dt, int *nt);
+/* Read traces [first, first+count) of a SEG-Y file: samples IBM-decoded like
+ * Kirchhoff.c:1284-1290 (ieee != 0 reads format-5 IEEE instead), coords gets
+ * header words 18-21 (sx,sy,gx,gy), extra (may be NULL) gets header words
+ * 47-48 (bytes 189-196: iline/xline, Kirchhoff.c:537-538). */
+int  ktm_segy_read(const char *path, int64_t first, int64_t count, int ieee,
+                   float *samples, int32_t *coords, int32_t *extra);
+/* Write a SEG-Y file from IEEE samples and full 240-byte headers. */
+int  ktm_segy_write(const char *path, const uint8_t *file_header3600,
+                    const uint8_t *trace_headers, const float *samples,
+                    int64_t ntr, int ns, int ieee);
+/* build_vel_model (Kirchhoff.c:449-626): velocity SEG-Y -> vel[il][xl][nsm]
+ * (not squared; zeros are counted in *nzero like the check at 614-615). */
+int  ktm_segy_read_velocity(const char *path, const ktm_config *cfg, float *vel,
+                            int *nsm, float *dtm, int64_t *nzero);
+/* Headers of the image output (Kirchhoff.c:729-814, 1437-1590). fix_count
+ * writes at byte 3212 the trace count the reference meant to write. */
+void ktm_segy_file_header(const ktm_config *cfg, int ns, float dt, int fix_count,
+                          uint8_t out[3600]);
+void ktm_segy_image_trace_header(const ktm_config *cfg, int ns, float dt, int job,
+                                 int xc, int yc, int ih, uint8_t out[240]);
+
+/* ---- reference-signature shim (Kirchhoff.c:92, 215-373, 1104) ---- */
+
+/* Does main()'s setup: count_size_file on input_data, create_aliasfilter,
+ * build_vel_model on input_vel, mkdir image_dir, header files.  vel_path may
+ * be NULL with vel (host, [il][xl][nsm]) given instead. */
+int  ktm_globals_init(const ktm_config *cfg, const char *input_data,
+                      const char *input_vel, const float *vel, int nsm, float dtm,
+                      const char *image_dir);
+/* kirchhoff(job_id): migrates the job's slab on the GPU from the input file
+ * and writes image_dir/image<job_id>.su byte-compatible with Kirchhoff.c. */
+void kirchhoff(int job_id);
+/* Status of the last kirchhoff() call (the reference exits instead). */
+int  ktm_last_status(void);
+void ktm_globals_free(void);
+
+/* ---- seeded synthetic surveys (device), for tests and benchmarks ---- */
+
+/* traces[ntr][ns] = sum of Ricker wavelets (peak frequency f0) on the
+ * double-square-root diffraction curves of nscat point scatterers
+ * scat[nscat][4] = (x, y, t0, amplitude) in the grid frame of geom (from
+ * ktm_geometry), with v = v0 + vt*t0 + vx*x at each scatterer. */
+int ktm_synth_traces(const ktm_plan *plan, const ktm_trace_geom *geom, int64_t ntr,
+                     const float *scat, int nscat, float f0, float v0, float vt, float vx,
+                     float *traces, void *stream);
+/* traces[i] += sigma * N(0,1), the normal deviate a function of (seed, i). */
+int ktm_synth_noise(float *traces, int64_t n, float sigma, uint64_t seed, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KTM_H */

@@ -1,0 +1,228 @@
+"""CPU oracle of the Kirchhoff migration path -- TEST INFRASTRUCTURE ONLY.
+
+ctypes wrapper of ``oracle/build/libktm_oracle.so`` (built from
+``oracle/ktm_oracle.c`` by ``oracle/Makefile``), a from-scratch C restatement
+of ``kirchhoff()`` in Kirchhoff.c with the reference's exact float/double
+promotions.  Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg may use it, and only as the checker or the timed CPU
+baseline -- never as the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "libktm_oracle.so")
+LIB_REFFLAGS_PATH = os.path.join(_HERE, "build", "libktm_oracle_refflags.so")
+
+FP32, FP64_DATA, FP64_FULL = 0, 1, 2
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("start_iline", C.c_int32), ("end_iline", C.c_int32),
+        ("start_xline", C.c_int32), ("end_xline", C.c_int32),
+        ("inc_iline", C.c_double), ("inc_xline", C.c_double),
+        ("dh", C.c_double), ("nh", C.c_int32), ("pad0", C.c_int32),
+        ("oh", C.c_double), ("ox", C.c_double), ("oy", C.c_double),
+        ("angle_e_to_n", C.c_double), ("iline_distance", C.c_double),
+        ("xline_distance", C.c_double), ("aperture", C.c_double),
+        ("anti_alias_cutoff", C.c_double),
+        ("nvel_ilines", C.c_int32), ("nvel_xlines", C.c_int32),
+        ("dvel_ilines", C.c_int32), ("dvel_xlines", C.c_int32),
+        ("ovel_ilines", C.c_int32), ("ovel_xlines", C.c_int32),
+        ("ntblock", C.c_int32), ("jobs_x", C.c_int32), ("jobs_y", C.c_int32),
+        ("image_origin_zero", C.c_int32), ("vel_lower_clamp", C.c_int32),
+        ("pad1", C.c_int32), ("scale_xy", C.c_double),
+    ]
+
+
+class OrcGrid(C.Structure):
+    _fields_ = [("Nx", C.c_int32), ("Ny", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32),
+                ("no_job", C.c_int32), ("sinA", C.c_float), ("cosA", C.c_float)]
+
+
+_vp, _i64, _i32, _f32 = C.c_void_p, C.c_int64, C.c_int, C.c_float
+_PROTOS = {
+    "orc_grid_init": (None, [C.POINTER(OrcParams), C.POINTER(OrcGrid)]),
+    "orc_ibm_block": (None, [_vp, _i64, _vp]),
+    "orc_alias_filter": (_i32, [C.POINTER(OrcParams), _i32, _f32, _vp]),
+    "orc_trace_prep_fp32": (None, [_vp, _i32, _f32, _i32, _vp]),
+    "orc_trace_prep_fp64": (None, [_vp, _i32, _f32, _i32, _vp]),
+    "orc_trace_geometry": (_i32, [C.POINTER(OrcParams), C.POINTER(OrcGrid), _vp, _vp]),
+    "orc_migrate_job": (_i64, [C.POINTER(OrcParams), _i32, _f32, _i32, _f32, _i32, _vp, _vp,
+                               _i64, _vp, _i32, _vp, _vp]),
+    "orc_migrate_all": (_i64, [C.POINTER(OrcParams), _i32, _f32, _i32, _f32, _vp, _vp, _i64,
+                               _vp, _i32, _vp, _vp, _i32]),
+    "orc_poststack_fp32": (None, [_i32, _f32, _vp, _vp, _vp, _i64, _vp]),
+    "orc_poststack_fp64": (None, [_i32, _f32, _vp, _vp, _vp, _i64, _vp]),
+    "orc_segy_trace_header": (None, [C.POINTER(OrcParams), C.POINTER(OrcGrid), _i32, _f32, _i32,
+                                     _i32, _i32, _i32, _i32, _vp]),
+    "orc_segy_file_header": (None, [C.POINTER(OrcParams), C.POINTER(OrcGrid), _i32, _f32, _i32, _vp]),
+    "orc_count_size": (_i32, [_vp, _i64, C.POINTER(_i32), C.POINTER(_f32), C.POINTER(_i32)]),
+}
+
+_libs: dict = {}
+
+
+def lib(refflags: bool = False) -> C.CDLL:
+    path = LIB_REFFLAGS_PATH if refflags else LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run `make -C oracle`")
+        L = C.CDLL(path)
+        for name, (res, args) in _PROTOS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _libs[path] = L
+    return _libs[path]
+
+
+def _vpn(a) -> C.c_void_p:
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def params(cfg) -> OrcParams:
+    """OrcParams from a random_amd Config (or any object with the same fields)."""
+    p = OrcParams()
+    for name, _ in OrcParams._fields_:
+        if name.startswith("pad"):
+            continue
+        setattr(p, name, getattr(cfg, name))
+    return p
+
+
+def grid(cfg) -> OrcGrid:
+    g = OrcGrid()
+    p = params(cfg)
+    lib().orc_grid_init(C.byref(p), C.byref(g))
+    return g
+
+
+def ibm_to_float(raw: np.ndarray) -> np.ndarray:
+    """raw: uint8 bytes in file order (multiple of 4) -> float32."""
+    raw = np.ascontiguousarray(raw, np.uint8)
+    out = np.empty(raw.size // 4, np.float32)
+    lib().orc_ibm_block(_vpn(raw), out.size, _vpn(out))
+    return out
+
+
+def alias_filter(cfg, ns: int, dt: float):
+    aa = np.zeros(ns, np.float32)
+    p = params(cfg)
+    tm = lib().orc_alias_filter(C.byref(p), ns, dt, _vpn(aa))
+    return tm, aa
+
+
+def trace_prep(tr: np.ndarray, dt: float, tm: int, fp64: bool = False) -> np.ndarray:
+    tr = np.ascontiguousarray(tr, np.float32)
+    ns = tr.size
+    if fp64:
+        P = np.zeros(ns + 2 * tm, np.float64)
+        lib().orc_trace_prep_fp64(_vpn(tr), ns, dt, tm, _vpn(P))
+    else:
+        P = np.zeros(ns + 2 * tm, np.float32)
+        lib().orc_trace_prep_fp32(_vpn(tr), ns, dt, tm, _vpn(P))
+    return P
+
+
+def filtered_trace(P: np.ndarray, tm: int, ns: int) -> np.ndarray:
+    """F[T] = -2 P[T+tm] + P[T+tm-2] + P[T+tm+2] (Kirchhoff.c:1392), evaluated in
+    double and rounded to the dtype of P."""
+    Pd = P.astype(np.float64)
+    T = np.arange(ns)
+    F = -2.0 * Pd[T + tm] + Pd[T + tm - 2] + Pd[T + tm + 2]
+    return F.astype(P.dtype)
+
+
+def trace_geometry(cfg, coords: np.ndarray):
+    """(geo float32 [ntr][6], off int32 [ntr]) for header words [ntr][4]."""
+    p, g = params(cfg), grid(cfg)
+    coords = np.ascontiguousarray(coords, np.int32)
+    geo = np.zeros((coords.shape[0], 6), np.float32)
+    off = np.zeros(coords.shape[0], np.int32)
+    for i in range(coords.shape[0]):
+        off[i] = lib().orc_trace_geometry(C.byref(p), C.byref(g), _vpn(coords[i]), _vpn(geo[i]))
+    return geo, off
+
+
+def migrate(cfg, ns, dt, traces, coords, vel, mode=FP32, nthreads=None, dtm=None,
+            refflags=False):
+    """All jobs into the padded image [nx*jobs_x][ny*jobs_y][nh][ns].
+    Returns (km, fold, contributions)."""
+    p, g = params(cfg), grid(cfg)
+    traces = np.ascontiguousarray(traces, np.float32)
+    coords = np.ascontiguousarray(coords, np.int32)
+    vel = np.ascontiguousarray(vel, np.float32)
+    shape = (g.nx * cfg.jobs_x, g.ny * cfg.jobs_y, cfg.nh, ns)
+    dtype = np.float32 if mode == FP32 else np.float64
+    km = np.zeros(shape, dtype)
+    fold = np.zeros(shape, dtype)
+    nthreads = nthreads or os.cpu_count() or 1
+    n = lib(refflags).orc_migrate_all(C.byref(p), ns, dt, vel.shape[2], dt if dtm is None else dtm,
+                                      _vpn(traces), _vpn(coords), traces.shape[0], _vpn(vel), mode,
+                                      _vpn(km), _vpn(fold), nthreads)
+    if n < 0:
+        raise ValueError("oracle: velocity index below the model")
+    return km, fold, int(n)
+
+
+def migrate_job(cfg, ns, dt, job, traces, coords, vel, mode=FP32, dtm=None):
+    p, g = params(cfg), grid(cfg)
+    traces = np.ascontiguousarray(traces, np.float32)
+    coords = np.ascontiguousarray(coords, np.int32)
+    vel = np.ascontiguousarray(vel, np.float32)
+    shape = (g.nx, g.ny, cfg.nh, ns)
+    dtype = np.float32 if mode == FP32 else np.float64
+    km = np.zeros(shape, dtype)
+    fold = np.zeros(shape, dtype)
+    n = lib().orc_migrate_job(C.byref(p), ns, dt, vel.shape[2], dt if dtm is None else dtm, job,
+                              _vpn(traces), _vpn(coords), traces.shape[0], _vpn(vel), mode,
+                              _vpn(km), _vpn(fold))
+    if n < 0:
+        raise ValueError("oracle: bad job or velocity index")
+    return km, fold, int(n)
+
+
+def poststack(km, fold, dt, anti_alias):
+    """Kirchhoff.c:1423-1479 per trace (last axis); float32 or float64 inputs."""
+    ns = km.shape[-1]
+    aa = np.ascontiguousarray(anti_alias, np.float32)
+    if km.dtype == np.float32:
+        k = np.ascontiguousarray(km.reshape(-1, ns))
+        f = np.ascontiguousarray(fold.reshape(-1, ns), np.float32)
+        out = np.zeros_like(k)
+        lib().orc_poststack_fp32(ns, dt, _vpn(aa), _vpn(k), _vpn(f), k.shape[0], _vpn(out))
+    else:
+        k = np.ascontiguousarray(km.reshape(-1, ns), np.float64)
+        f = np.ascontiguousarray(fold.reshape(-1, ns), np.float64)
+        out = np.zeros_like(k)
+        lib().orc_poststack_fp64(ns, dt, _vpn(aa), _vpn(k), _vpn(f), k.shape[0], _vpn(out))
+    return out.reshape(km.shape)
+
+
+def segy_trace_header(cfg, ns, dt, job, xc, yc, ih, tracl) -> bytes:
+    p, g = params(cfg), grid(cfg)
+    out = np.zeros(240, np.uint8)
+    lib().orc_segy_trace_header(C.byref(p), C.byref(g), ns, dt, job, xc, yc, ih, tracl, _vpn(out))
+    return out.tobytes()
+
+
+def segy_file_header(cfg, ns, dt, fix_count=0) -> bytes:
+    p, g = params(cfg), grid(cfg)
+    out = np.zeros(3600, np.uint8)
+    lib().orc_segy_file_header(C.byref(p), C.byref(g), ns, dt, fix_count, _vpn(out))
+    return out.tobytes()
+
+
+def count_size(head: bytes, length: int):
+    b = np.frombuffer(head[:3720].ljust(3720, b"\0"), np.uint8).copy()
+    ns, dt, nt = C.c_int(), C.c_float(), C.c_int()
+    rc = lib().orc_count_size(_vpn(b), length, C.byref(ns), C.byref(dt), C.byref(nt))
+    if rc != 0:
+        raise ValueError("file too short")
+    return ns.value, dt.value, nt.value

@@ -1,0 +1,932 @@
+/*
+ * ktm_device.hip -- gfx950 kernels and the device half of the C ABI of the
+ * Kirchhoff prestack time-migration engine.
+ *
+ * Kernels (every one replaces a loop of Kirchhoff.c's kirchhoff()):
+ *   k_geometry      per-trace rotation / CMP / offset class   Kirchhoff.c:1296-1310
+ *   k_prep_ref      FP32 double integral + trange=2 taps       Kirchhoff.c:1312-1328,1391-1392
+ *   k_prep_closed   same operator as a 3-tap filter (FP32)     (exact rewrite of 1312-1392)
+ *   k_quantize      BF16/FP16 storage with per-trace scale
+ *   k_migrate       the image loop, Kirchhoff.c:1332-1404: one workgroup owns a
+ *                   tile of image columns (one wave per column, all ns samples
+ *                   in registers), streams the traces of a block through LDS in
+ *                   trace order and accumulates F(floor(dsr/dt)) per sample
+ *   k_poststack     derivative/fold + low-pass, Kirchhoff.c:1457-1479
+ *   k_demigrate     exact transpose of k_migrate (CLOSED prep)
+ *   k_synth_*       seeded synthetic surveys
+ *
+ * Build flags: -ffp-contract=off (no implicit FMA: every product and sum
+ * that the reference rounds separately is rounded separately here; FMAs are
+ * written explicitly where intended) and the default correctly-rounded FP32
+ * divide/sqrt; packed-FP32 ops are disabled.
+ */
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "ktm_internal.h"
+
+#define KTM_WAVE 64
+
+/* ------------------------------------------------------------------------ */
+/* plan                                                                      */
+/* ------------------------------------------------------------------------ */
+struct ktm_plan {
+    ktm_config cfg;
+    ktm_tables tb;
+    int device;
+    int64_t row;            /* prepared trace row (elements), multiple of 8, >= ns+1 */
+    float *d_t2;
+    int32_t *d_t1;
+    float *d_colx, *d_coly;
+    int32_t *d_velx, *d_vely;
+    float *d_slab_cx, *d_slab_cy;
+    double *d_slab_thr;
+    float *d_vel2;          /* [nvel_il][nvel_xl][nsm], squared like Kirchhoff.c:617 */
+    float *d_w;             /* [ns+2] CLOSED weights */
+    double *d_hconv;        /* [ns]   post-stack kernel */
+};
+
+#define HIP_TRY(expr)                                  \
+    do {                                               \
+        hipError_t e_ = (expr);                        \
+        if (e_ != hipSuccess) return KTM_ERR_DEVICE;   \
+    } while (0)
+
+/* ------------------------------------------------------------------------ */
+/* per-trace geometry, Kirchhoff.c:1296-1310 (and Kirchhoff_GHTest1.c:       */
+/* 1302-1315 when scale_xy > 0)                                              */
+/* ------------------------------------------------------------------------ */
+struct GeoArgs {
+    double ox, oy, scale, oh, dh;
+    float sinA, cosA;
+};
+
+__device__ __forceinline__ void rotate_point(const GeoArgs &g, uint32_t hx, uint32_t hy,
+                                             float &xt, float &yt)
+{
+    if (g.scale == 0.0) {
+        /* htonl() gives uint32_t; minus the double origin; into an int */
+        const int i = (int)((double)hx - g.ox);
+        const int j = (int)((double)hy - g.oy);
+        xt = g.cosA * (float)i + g.sinA * (float)j;
+        yt = -g.sinA * (float)i + g.cosA * (float)j;
+    } else {
+        const int i = (int)((double)hx - g.ox / g.scale);
+        const int j = (int)((double)hy - g.oy / g.scale);
+        xt = (float)((double)(g.cosA * (float)i) * g.scale + (double)(g.sinA * (float)j) * g.scale);
+        yt = (float)((double)(-g.sinA * (float)i) * g.scale + (double)(g.cosA * (float)j) * g.scale);
+    }
+}
+
+__device__ __forceinline__ ktm_trace_geom trace_geometry(const GeoArgs &g, const int32_t *c)
+{
+    ktm_trace_geom r;
+    rotate_point(g, (uint32_t)c[0], (uint32_t)c[1], r.sxt, r.syt);
+    rotate_point(g, (uint32_t)c[2], (uint32_t)c[3], r.gxt, r.gyt);
+    const float ex = r.sxt - r.gxt, ey = r.syt - r.gyt;
+    const float h2 = (float)(0.25 * (double)(ex * ex + ey * ey));
+    r.cmpx = (float)(0.5 * (double)(r.sxt + r.gxt));
+    r.cmpy = (float)(0.5 * (double)(r.syt + r.gyt));
+    r.off = (int)floor((2.0 * sqrt((double)h2) - g.oh) / g.dh);
+    r.scale = 1.0f;
+    return r;
+}
+
+__global__ void k_geometry(GeoArgs g, const int32_t *__restrict__ coords, int64_t ntr,
+                           ktm_trace_geom *__restrict__ geom)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntr) return;
+    geom[i] = trace_geometry(g, coords + 4 * i);
+}
+
+/* ------------------------------------------------------------------------ */
+/* trace preparation                                                         */
+/* ------------------------------------------------------------------------ */
+
+/* REFERENCE mode: one lane per trace runs the two FP32 running sums of
+ * Kirchhoff.c:1314-1324 in a single pass, keeping the last five values of
+ * the double integral p[] in registers, and emits
+ *   F[T] = (float)(-2.0*p[T] + p[T-2] + p[T+2])         (Kirchhoff.c:1392)
+ * as soon as p[T+2] exists.  p[-1] = p[1] and p[-2] = p[2] are the mirror pad
+ * of Kirchhoff.c:1327-1328.  No scratch buffer is needed. */
+__global__ void k_prep_ref(const float *__restrict__ traces, int64_t ntr, int ns, int64_t row,
+                           float dt, const float *__restrict__ t2, float *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntr) return;
+    const float *tr = traces + i * (int64_t)ns;
+    float *F = out + i * row;
+    const float last = tr[ns - 1];
+    float q = tr[0];        /* integral 1 at t */
+    float p = q;            /* integral 2 at t */
+    float pm1 = 0.0f, pm2 = 0.0f, pm3 = 0.0f, pm4 = 0.0f;
+    for (int t = 1; t <= ns + 1; ++t) {
+        if (t < ns)
+            q = (float)((double)q + (double)tr[t] / (4.0 * (double)t2[t]));
+        else
+            q = q + last / dt / dt / (float)t / (float)t;
+        pm4 = pm3; pm3 = pm2; pm2 = pm1; pm1 = p;
+        p = p + q;
+        /* window: pm4..p = p[t-4]..p[t] */
+        if (t == 2)
+            F[0] = (float)(-2.0 * (double)pm2 + (double)p + (double)p);
+        else if (t == 3)
+            F[1] = (float)(-2.0 * (double)pm2 + (double)pm2 + (double)p);
+        else if (t >= 4)
+            F[t - 2] = (float)(-2.0 * (double)pm2 + (double)pm4 + (double)p);
+    }
+    for (int64_t t = ns; t < row; ++t) F[t] = 0.0f;
+}
+
+/* CLOSED mode: F as the 3-tap filter of x[t] = trace[min(t,ns-1)] * w[t].
+ * Derivation from Kirchhoff.c:1314-1328 (exact arithmetic): with
+ * Q[k] = sum_{j<=k} x[j] and p[k] = sum_{j<=k} Q[j],
+ *   T >= 2 : p[T+2] - 2p[T] + p[T-2] = x[T] + 2x[T+1] + x[T+2]
+ *   T == 1 : p[3] - p[1]           = 2x[0] + 2x[1] + 2x[2] + x[3]
+ *   T == 0 : 2(p[2] - p[0])        = 4x[0] + 4x[1] + 2x[2]
+ * Shared by the F32 prep kernel and the BF16/FP16 staging in k_migrate so the
+ * two storage modes differ only by quantisation. */
+__device__ __forceinline__ float closed_tap(int T, float x0, float x1, float x2, float x3)
+{
+    /* for T >= 2, x0..x2 are x[T], x[T+1], x[T+2]; for T < 2 they are x[0..3] */
+    if (T >= 2) return (x0 + 2.0f * x1) + x2;
+    if (T == 1) return ((2.0f * x0 + 2.0f * x1) + 2.0f * x2) + x3;
+    return (4.0f * x0 + 4.0f * x1) + 2.0f * x2;
+}
+
+__global__ void k_prep_closed(const float *__restrict__ traces, int64_t ntr, int ns, int64_t row,
+                              const float *__restrict__ w, float *__restrict__ out)
+{
+    const int64_t i = blockIdx.y;
+    const int T = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntr || T >= row) return;
+    const float *tr = traces + i * (int64_t)ns;
+    float v = 0.0f;
+    if (T < ns) {
+        float x[4];
+        const int b = T >= 2 ? T : 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int t = b + k;
+            x[k] = t <= ns + 1 ? tr[t < ns ? t : ns - 1] * w[t] : 0.0f;
+        }
+        v = closed_tap(T, x[0], x[1], x[2], x[3]);
+    }
+    out[i * row + T] = v;
+}
+
+/* BF16 (round to nearest even) and FP16 storage helpers */
+__device__ __forceinline__ uint16_t f32_to_bf16(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u); /* NaN */
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ uint16_t f32_to_f16(float f)
+{
+    _Float16 h = (_Float16)f;
+    uint16_t u;
+    __builtin_memcpy(&u, &h, 2);
+    return u;
+}
+__device__ __forceinline__ float f16_to_f32(uint16_t u)
+{
+    _Float16 h;
+    __builtin_memcpy(&h, &u, 2);
+    return (float)h;
+}
+
+/* One wave per trace: scale = max |tr| (1 for an all-zero trace), stored
+ * value = tr * (1/scale) rounded to BF16 or FP16. */
+template <int DT>
+__global__ void k_quantize(const float *__restrict__ traces, int64_t ntr, int ns, int64_t row,
+                           uint16_t *__restrict__ q, ktm_trace_geom *__restrict__ geom)
+{
+    const int64_t i = (int64_t)blockIdx.x * (blockDim.x / KTM_WAVE) + (threadIdx.x / KTM_WAVE);
+    const int lane = threadIdx.x & (KTM_WAVE - 1);
+    if (i >= ntr) return;
+    const float *tr = traces + i * (int64_t)ns;
+    float m = 0.0f;
+    for (int t = lane; t < ns; t += KTM_WAVE) m = fmaxf(m, fabsf(tr[t]));
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) m = fmaxf(m, __shfl_xor(m, s, KTM_WAVE));
+    const float scale = m > 0.0f ? m : 1.0f;
+    const float inv = 1.0f / scale;
+    uint16_t *o = q + i * row;
+    for (int64_t t = lane; t < row; t += KTM_WAVE) {
+        const float v = t < ns ? tr[t] * inv : 0.0f;
+        o[t] = DT == KTM_TRACE_BF16 ? f32_to_bf16(v) : f32_to_f16(v);
+    }
+    if (lane == 0) geom[i].scale = scale;
+}
+
+/* ------------------------------------------------------------------------ */
+/* migration                                                                 */
+/* ------------------------------------------------------------------------ */
+struct MigArgs {
+    const void *prep;                /* [ntr][row] float, or uint16 for BF16/FP16 */
+    const ktm_trace_geom *geom;
+    int64_t ntr, row;
+    float *km, *fold, *comp;
+    const float *vel2;
+    const int32_t *t1, *velx, *vely;
+    const float *t2, *colx, *coly, *slab_cx, *slab_cy, *w;
+    const double *slab_thr;
+    unsigned long long *counter;
+    double aperture;
+    float dt, rdt;
+    int ns, nh, nxp, nyp, nx, ny, jobs_y, nvel_xl, nsm;
+    int tiles_x, ntiles, tx_cols, ty_cols;
+    int rx0, ry0, rx1, ry1;          /* migrated region [rx0,rx1) x [ry0,ry1) */
+};
+
+/* Reference semantics of one traveltime index (Kirchhoff.c:1381-1384):
+ * FP32 true divisions and sums, the two square roots in double and summed in
+ * double, rounded to float, FP32 division by dt, floor.  Returns ns when the
+ * sample falls past the trace (the T<ns test of Kirchhoff.c:1387). */
+__device__ __forceinline__ int exact_T(float dIS2, float dIR2, float t2, float v2, float dt, int ns)
+{
+    const float s1 = t2 + dIS2 / v2;
+    const float s2 = t2 + dIR2 / v2;
+    const float dsr = (float)(__builtin_sqrt((double)s1) + __builtin_sqrt((double)s2));
+    const float qq = dsr / dt;
+    return qq < (float)ns ? (int)qq : ns;
+}
+
+/* Relative bound on |fast q - exact q| (see k_migrate): 11 units of 2^-24
+ * from the analysis, rounded up to 1e-6. */
+#define KTM_FAST_EPS 1.0e-6f
+
+template <int K, int W, int B, bool KAHAN, int SRC>
+__global__ void __launch_bounds__(8 * KTM_WAVE)
+k_migrate(const MigArgs a)
+{
+    constexpr int NCOL = 8 / W;             /* image columns per workgroup */
+    constexpr int NW = NCOL * W;            /* waves per workgroup (8) */
+    constexpr int NT = NW * KTM_WAVE;
+    extern __shared__ float4 lds4[];
+    const int ldl = (int)a.row;             /* LDS row = prepared row (floats) */
+    float *sF = reinterpret_cast<float *>(lds4);            /* [B][ldl] */
+    float *sG = sF + B * ldl;                               /* [B][8]   */
+    int *sSel = reinterpret_cast<int *>(sG + B * 8);        /* [NT]     */
+    int *sCnt = sSel + NT;                                  /* [NW]     */
+    float *sW = reinterpret_cast<float *>(sCnt + NW);       /* [ns+2] (BF16/F16 only) */
+
+    const int tid = threadIdx.x, lane = tid & (KTM_WAVE - 1), wv = tid / KTM_WAVE;
+    /* XCD-aware tile order: consecutive workgroup ids go to different XCDs,
+     * so give each XCD a contiguous range of tiles (shared traces in its L2). */
+    const int per = (a.ntiles + 7) >> 3;
+    const int tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (tile >= a.ntiles) return;
+    const int h = blockIdx.y;
+    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+    const int cw = wv / W, seg = wv % W;    /* column within tile, time segment */
+    const int X = a.rx0 + tx * a.tx_cols + cw % a.tx_cols;
+    const int Y = a.ry0 + ty * a.ty_cols + cw / a.tx_cols;
+    const bool col_ok = X < a.rx1 && Y < a.ry1;
+    const int Xc = X < a.rx1 ? X : a.rx1 - 1, Yc = Y < a.ry1 ? Y : a.ry1 - 1;
+    const float IX = a.colx[Xc], IY = a.coly[Yc];
+    const int job = (Xc / a.nx) * a.jobs_y + (Yc / a.ny);
+    const float scx = a.slab_cx[job], scy = a.slab_cy[job];
+    const double sthr = a.slab_thr[job];
+    const int ns = a.ns;
+    const float nsf = (float)ns, nsf1 = (float)(ns + 1);
+
+    /* conservative tile test: the CMP must be within aperture of some
+     * column of the tile (exact per-column gates follow) */
+    const int x0 = a.rx0 + tx * a.tx_cols, x1 = min(x0 + a.tx_cols, a.rx1) - 1;
+    const int y0 = a.ry0 + ty * a.ty_cols, y1 = min(y0 + a.ty_cols, a.ry1) - 1;
+    const float ccx = 0.5f * (a.colx[x0] + a.colx[x1]), ccy = 0.5f * (a.coly[y0] + a.coly[y1]);
+    const float hx = 0.5f * fabsf(a.colx[x1] - a.colx[x0]);
+    const float hy = 0.5f * fabsf(a.coly[y1] - a.coly[y0]);
+    const float reach = (float)a.aperture * 1.0001f + sqrtf(hx * hx + hy * hy) + 1.0f;
+    const float reach2 = reach * reach;
+
+    if (SRC != KTM_TRACE_F32)
+        for (int t = tid; t < ns + 2; t += NT) sW[t] = a.w[t];
+
+    /* per-sample constants of this wave's column segment */
+    const int64_t pix = (((int64_t)Xc * a.nyp + Yc) * a.nh + h) * ns;
+    const int64_t vbase = ((int64_t)a.velx[Xc] * a.nvel_xl + a.vely[Yc]) * a.nsm;
+    float r[K], t2v[K], v2v[K], acc[K], cmp[K];
+    int cnt[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int t = seg * K * KTM_WAVE + k * KTM_WAVE + lane;
+        cnt[k] = 0;
+        cmp[k] = 0.0f;
+        if (t < ns) {
+            const float v2 = a.vel2[vbase + a.t1[t]];
+            v2v[k] = v2;
+            r[k] = 1.0f / v2;
+            t2v[k] = a.t2[t];
+            acc[k] = col_ok ? a.km[pix + t] : 0.0f;
+            if (KAHAN && col_ok) cmp[k] = a.comp[pix + t];
+        } else {
+            v2v[k] = 1.0f;
+            r[k] = 0.0f;
+            t2v[k] = 3.0e38f;   /* q becomes huge: T = ns, nothing added */
+            acc[k] = 0.0f;
+        }
+    }
+
+    uint32_t npass = 0;
+    const int n4 = ldl / 4;
+    for (int64_t c0 = 0; c0 < a.ntr; c0 += NT) {
+        /* ---- select the traces of this chunk that can reach the tile, in order ---- */
+        const int64_t i = c0 + tid;
+        bool pred = false;
+        if (i < a.ntr) {
+            const ktm_trace_geom g = a.geom[i];
+            const float dx = g.cmpx - ccx, dy = g.cmpy - ccy;
+            pred = g.off == h && dx * dx + dy * dy <= reach2;
+        }
+        const uint64_t m = __ballot(pred);
+        const int pre = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) sCnt[wv] = __popcll(m);
+        __syncthreads();
+        int base = 0, total = 0;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+            const int v = sCnt[q];
+            base += q < wv ? v : 0;
+            total += v;
+        }
+        if (pred) sSel[base + pre] = tid;
+        __syncthreads();
+
+        for (int b0 = 0; b0 < total; b0 += B) {
+            const int nb = min(B, total - b0);
+            /* ---- stage nb traces (filtered samples + geometry) into LDS ---- */
+            for (int j = 0; j < nb; ++j) {
+                const int64_t ti = c0 + sSel[b0 + j];
+                if (SRC == KTM_TRACE_F32) {
+                    const float4 *src = reinterpret_cast<const float4 *>(
+                        reinterpret_cast<const float *>(a.prep) + ti * a.row);
+                    for (int c = tid; c < n4; c += NT)
+                        reinterpret_cast<float4 *>(sF + j * ldl)[c] = src[c];
+                } else {
+                    const uint16_t *src = reinterpret_cast<const uint16_t *>(a.prep) + ti * a.row;
+                    const float sc = a.geom[ti].scale;
+                    for (int c = tid; c < n4; c += NT) {
+                        const int T0 = 4 * c;
+                        float x[8];
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            const int t = T0 + k;
+                            const int tt = t < ns ? t : ns - 1;
+                            const float sv = SRC == KTM_TRACE_BF16 ? bf16_to_f32(src[tt])
+                                                                   : f16_to_f32(src[tt]);
+                            x[k] = t <= ns + 1 ? (sc * sv) * sW[t] : 0.0f;
+                        }
+                        float4 o;
+                        float *op = reinterpret_cast<float *>(&o);
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int T = T0 + u;
+                            float v = 0.0f;
+                            if (T < ns) {
+                                if (T >= 2) v = closed_tap(T, x[u], x[u + 1], x[u + 2], 0.0f);
+                                else v = closed_tap(T, x[0], x[1], x[2], x[3]);
+                            }
+                            op[u] = v;
+                        }
+                        reinterpret_cast<float4 *>(sF + j * ldl)[c] = o;
+                    }
+                }
+            }
+            if (tid < nb * 2) {
+                const int j = tid >> 1, half = tid & 1;
+                const int64_t ti = c0 + sSel[b0 + j];
+                reinterpret_cast<float4 *>(sG + j * 8)[half] =
+                    reinterpret_cast<const float4 *>(a.geom + ti)[half];
+            }
+            __syncthreads();
+
+            if (col_ok) {
+                for (int j = 0; j < nb; ++j) {
+                    const float *g = sG + j * 8;
+                    const float sxt = g[0], syt = g[1], gxt = g[2], gyt = g[3];
+                    const float cmx = g[4], cmy = g[5];
+                    /* slab prefilter, Kirchhoff.c:1334-1338 */
+                    const float ex = scx - cmx, ey = scy - cmy;
+                    const float ds = (float)__builtin_sqrt((double)(ex * ex + ey * ey));
+                    /* column gates and distances, Kirchhoff.c:1350-1354 */
+                    const float ax = IX - sxt, ay = IY - syt, bx = IX - gxt, by = IY - gyt;
+                    const float cx = IX - cmx, cy = IY - cmy;
+                    const float dIS2 = ax * ax + ay * ay;
+                    const float dIR2 = bx * bx + by * by;
+                    const float dc = (float)__builtin_sqrt((double)(cx * cx + cy * cy));
+                    const bool pass = (double)ds < sthr && (double)dc < a.aperture;
+                    if (!__builtin_amdgcn_readfirstlane((int)pass)) continue;
+                    ++npass;
+                    const float *Fr = sF + j * ldl;
+                    /* Hot loop, Kirchhoff.c:1374-1399.  Fast path: q = dsr/dt from
+                     * FMA (one reciprocal of v^2 per sample, precomputed), the
+                     * hardware square root and a reciprocal of dt.  Its distance
+                     * to the reference value is at most KTM_FAST_EPS*q (+1e-6);
+                     * whenever an integer lies within that distance the lane
+                     * recomputes T with the reference arithmetic, so T is
+                     * always the reference's. */
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        const float a1 = __builtin_fmaf(dIS2, r[k], t2v[k]);
+                        const float a2 = __builtin_fmaf(dIR2, r[k], t2v[k]);
+                        const float d = __builtin_amdgcn_sqrtf(a1) + __builtin_amdgcn_sqrtf(a2);
+                        const float q = d * a.rdt;
+                        int T = (int)fminf(q, nsf);
+                        const float dev = fabsf(q - __builtin_rintf(q));
+                        if (q < nsf1 && dev <= __builtin_fmaf(q, KTM_FAST_EPS, KTM_FAST_EPS))
+                            T = exact_T(dIS2, dIR2, t2v[k], v2v[k], a.dt, ns);
+                        const float f = Fr[T];
+                        if (KAHAN) {
+                            const float y = f - cmp[k];
+                            const float s = acc[k] + y;
+                            cmp[k] = (s - acc[k]) - y;
+                            acc[k] = s;
+                        } else {
+                            acc[k] = acc[k] + f;
+                        }
+                        cnt[k] += T < ns ? 1 : 0;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    if (col_ok) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int t = seg * K * KTM_WAVE + k * KTM_WAVE + lane;
+            if (t < ns) {
+                a.km[pix + t] = acc[k];
+                a.fold[pix + t] = a.fold[pix + t] + (float)cnt[k];
+                if (KAHAN) a.comp[pix + t] = cmp[k];
+            }
+        }
+        if (lane == 0 && seg == 0 && a.counter)
+            atomicAdd(a.counter, (unsigned long long)npass * (unsigned long long)ns);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* image reset and post-stack                                                */
+/* ------------------------------------------------------------------------ */
+__global__ void k_image_reset(float *km, float *fold, float *comp, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        km[i] = 0.0f;
+        fold[i] = 1.0f;
+        if (comp) comp[i] = 0.0f;
+    }
+}
+
+/* Kirchhoff.c:1457-1479 for PS_TR image traces per workgroup.  The low-pass
+ * multiplies the DFT by the even real spectrum anti_alias, which is the
+ * circular convolution out[t] = sum_u h[(t-u) mod ns] d[u] with h the
+ * (real) inverse DFT of anti_alias divided by ns; evaluated in FP64. */
+#define PS_TR 4
+#define PS_NT 256
+template <int SLOTS>
+__global__ void __launch_bounds__(PS_NT)
+k_poststack(const float *__restrict__ km, const float *__restrict__ fold, int64_t ntraces,
+            int ns, float dt, const double *__restrict__ hconv, float *__restrict__ out)
+{
+    extern __shared__ double lds_d[];
+    double *hh = lds_d;              /* [2*ns]: hh[m] = h[m mod ns]      */
+    double *dd = lds_d + 2 * ns;     /* [ns][PS_TR] derivative traces    */
+    const int64_t tr0 = (int64_t)blockIdx.x * PS_TR;
+    for (int m = threadIdx.x; m < 2 * ns; m += PS_NT) hh[m] = hconv[m < ns ? m : m - ns];
+    for (int e = threadIdx.x; e < ns * PS_TR; e += PS_NT) {
+        const int u = e / PS_TR, b = e - u * PS_TR;
+        const int64_t tr = tr0 + b;
+        double v = 0.0;
+        if (tr < ntraces && u >= 1) {
+            const float *k = km + tr * ns;
+            /* Kirchhoff.c:1460: a float expression stored into a double */
+            v = (double)(-dt * (float)u * (k[u] - k[u - 1]) * dt / fold[tr * ns + u]);
+        }
+        dd[u * PS_TR + b] = v;
+    }
+    __syncthreads();
+    double acc[SLOTS][PS_TR];
+#pragma unroll
+    for (int s = 0; s < SLOTS; ++s)
+#pragma unroll
+        for (int b = 0; b < PS_TR; ++b) acc[s][b] = 0.0;
+    for (int u = 0; u < ns; ++u) {
+        double dv[PS_TR];
+#pragma unroll
+        for (int b = 0; b < PS_TR; ++b) dv[b] = dd[u * PS_TR + b];
+#pragma unroll
+        for (int s = 0; s < SLOTS; ++s) {
+            const int t = s * PS_NT + threadIdx.x;
+            const double hv = t < ns ? hh[t - u + ns] : 0.0;
+#pragma unroll
+            for (int b = 0; b < PS_TR; ++b) acc[s][b] = __builtin_fma(hv, dv[b], acc[s][b]);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < SLOTS; ++s) {
+        const int t = s * PS_NT + threadIdx.x;
+        if (t < ns) {
+#pragma unroll
+            for (int b = 0; b < PS_TR; ++b) {
+                const int64_t tr = tr0 + b;
+                if (tr < ntraces) out[tr * ns + t] = t == 0 ? 0.0f : (float)acc[s][b];
+            }
+        }
+    }
+}
+
+
+/* ------------------------------------------------------------------------ */
+/* seeded synthetic surveys                                                  */
+/* ------------------------------------------------------------------------ */
+/* One workgroup per trace: the DSR diffraction time of every scatterer,
+ *   tau_j = sqrt((t0/2)^2 + |s-x_j|^2/v_j^2) + sqrt((t0/2)^2 + |g-x_j|^2/v_j^2),
+ * v_j = v0 + vt*t0_j + vx*x_j, in LDS; then each sample sums the Ricker
+ * wavelets (1 - 2 a) exp(-a), a = (pi f0 (t dt - tau_j))^2, of the scatterers
+ * within 2/f0 (the wavelet is below 1e-13 there), in scatterer order. */
+#define SYN_NT 256
+#define SYN_MAXSCAT 1024
+__global__ void __launch_bounds__(SYN_NT)
+k_synth_traces(const ktm_trace_geom *__restrict__ geom, int64_t ntr, int ns, float dt,
+               const float *__restrict__ scat, int nscat, float f0, float v0, float vt, float vx,
+               float *__restrict__ traces)
+{
+    __shared__ float s_tau[SYN_MAXSCAT], s_amp[SYN_MAXSCAT];
+    const int64_t i = blockIdx.x;
+    if (i >= ntr) return;
+    const ktm_trace_geom g = geom[i];
+    for (int j = threadIdx.x; j < nscat; j += SYN_NT) {
+        const double x = scat[4 * j], y = scat[4 * j + 1], t0 = scat[4 * j + 2];
+        const double v = (double)v0 + (double)vt * t0 + (double)vx * x;
+        const double h = 0.25 * t0 * t0, iv2 = 1.0 / (v * v);
+        const double ds2 = (g.sxt - x) * (g.sxt - x) + (g.syt - y) * (g.syt - y);
+        const double dg2 = (g.gxt - x) * (g.gxt - x) + (g.gyt - y) * (g.gyt - y);
+        s_tau[j] = (float)(sqrt(h + ds2 * iv2) + sqrt(h + dg2 * iv2));
+        s_amp[j] = scat[4 * j + 3];
+    }
+    __syncthreads();
+    const float win = 2.0f / f0;
+    const float pf = 3.14159265358979f * f0;
+    float *o = traces + i * (int64_t)ns;
+    for (int t = threadIdx.x; t < ns; t += SYN_NT) {
+        const float tt = (float)t * dt;
+        float acc = 0.0f;
+        for (int j = 0; j < nscat; ++j) {
+            const float d = tt - s_tau[j];
+            if (fabsf(d) < win) {
+                const float a = (pf * d) * (pf * d);
+                acc += s_amp[j] * (1.0f - 2.0f * a) * expf(-a);
+            }
+        }
+        o[t] = acc;
+    }
+}
+
+/* counter-based Gaussian noise: SplitMix64 of (seed, index) -> two uniforms
+ * -> Box-Muller; the value at an index does not depend on the launch shape */
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void k_synth_noise(float *traces, int64_t n, float sigma, uint64_t seed)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t r = splitmix64(seed ^ splitmix64((uint64_t)i));
+        const double u1 = ((double)(r >> 40) + 0.5) * (1.0 / 16777216.0);
+        const double u2 = (double)((r >> 16) & 0xffffffull) * (1.0 / 16777216.0);
+        const double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+        traces[i] += (float)(sigma * z);
+    }
+}
+
+extern "C" int ktm_synth_traces(const ktm_plan *p, const ktm_trace_geom *geom, int64_t ntr,
+                                const float *scat, int nscat, float f0, float v0, float vt,
+                                float vx, float *traces, void *stream)
+{
+    if (!p || ntr < 0 || nscat < 0 || nscat > SYN_MAXSCAT || !(f0 > 0.0f)) return KTM_ERR_ARG;
+    if (!ntr) return KTM_OK;
+    if (!geom || !traces || (nscat && !scat)) return KTM_ERR_ARG;
+    for (int64_t i0 = 0; i0 < ntr; i0 += 1 << 30) {
+        const int64_t n = ntr - i0 < (1 << 30) ? ntr - i0 : (1 << 30);
+        hipLaunchKernelGGL(k_synth_traces, dim3((unsigned)n), dim3(SYN_NT), 0, (hipStream_t)stream,
+                           geom + i0, n, p->tb.s.ns, p->tb.s.dt, scat, nscat, f0, v0, vt, vx,
+                           traces + i0 * p->tb.s.ns);
+    }
+    HIP_TRY(hipGetLastError());
+    return KTM_OK;
+}
+
+extern "C" int ktm_synth_noise(float *traces, int64_t n, float sigma, uint64_t seed, void *stream)
+{
+    if (n < 0 || (n && !traces)) return KTM_ERR_ARG;
+    if (!n) return KTM_OK;
+    hipLaunchKernelGGL(k_synth_noise, dim3(4096), dim3(256), 0, (hipStream_t)stream, traces, n,
+                       sigma, seed);
+    HIP_TRY(hipGetLastError());
+    return KTM_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* host glue                                                                 */
+/* ------------------------------------------------------------------------ */
+template <typename T>
+static int upload(T **dst, const T *src, size_t n)
+{
+    *dst = nullptr;
+    if (hipMalloc((void **)dst, sizeof(T) * (n ? n : 1)) != hipSuccess) return KTM_ERR_NOMEM;
+    if (n && hipMemcpy(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice) != hipSuccess)
+        return KTM_ERR_DEVICE;
+    return KTM_OK;
+}
+
+extern "C" void ktm_plan_destroy(ktm_plan *p)
+{
+    if (!p) return;
+    void *bufs[] = {p->d_t2, p->d_t1, p->d_colx, p->d_coly, p->d_velx, p->d_vely,
+                    p->d_slab_cx, p->d_slab_cy, p->d_slab_thr, p->d_vel2, p->d_w, p->d_hconv};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    ktm_tables_free(&p->tb);
+    free(p);
+}
+
+extern "C" int ktm_plan_create(ktm_plan **out, const ktm_config *cfg, int ns, float dt, int nsm,
+                               float dtm, const float *vel)
+{
+    if (!out || !cfg || !vel) return KTM_ERR_ARG;
+    *out = nullptr;
+    if (ns < 4 || ns > 3072) return KTM_ERR_UNSUPPORTED;
+    ktm_plan *p = (ktm_plan *)calloc(1, sizeof(ktm_plan));
+    if (!p) return KTM_ERR_NOMEM;
+    p->cfg = *cfg;
+    int st = ktm_tables_build(cfg, ns, dt, nsm, dtm, &p->tb);
+    if (st != KTM_OK) { free(p); return st; }
+    if (hipGetDevice(&p->device) != hipSuccess) { ktm_plan_destroy(p); return KTM_ERR_DEVICE; }
+    p->row = ((int64_t)ns + 1 + 7) / 8 * 8;
+    /* square the velocity model (Kirchhoff.c:617) and check the part the
+     * image uses: a velocity <= 0 (or NaN) would divide by zero at 1381 */
+    const ktm_tables *tb = &p->tb;
+    const size_t nv = (size_t)cfg->nvel_ilines * cfg->nvel_xlines * nsm;
+    float *v2 = (float *)malloc(sizeof(float) * nv);
+    if (!v2) { ktm_plan_destroy(p); return KTM_ERR_NOMEM; }
+    for (size_t k = 0; k < nv; ++k) v2[k] = vel[k] * vel[k];
+    int bad = 0;
+    for (int x = 0; x < tb->s.nxp && !bad; ++x)
+        for (int y = 0; y < tb->s.nyp && !bad; ++y) {
+            const float *col = vel + ((size_t)tb->velx[x] * cfg->nvel_xlines + tb->vely[y]) * nsm;
+            for (int t = 0; t < ns; ++t)
+                if (!(col[tb->t1[t]] > 0.0f) || !(col[tb->t1[t]] < 3.0e19f)) { bad = 1; break; }
+        }
+    if (bad) { free(v2); ktm_plan_destroy(p); return KTM_ERR_VELOCITY; }
+    st = upload(&p->d_vel2, v2, nv);
+    free(v2);
+    if (st == KTM_OK) st = upload(&p->d_t2, tb->t2, (size_t)ns);
+    if (st == KTM_OK) st = upload(&p->d_t1, tb->t1, (size_t)ns);
+    if (st == KTM_OK) st = upload(&p->d_colx, tb->colx, (size_t)tb->s.nxp);
+    if (st == KTM_OK) st = upload(&p->d_coly, tb->coly, (size_t)tb->s.nyp);
+    if (st == KTM_OK) st = upload(&p->d_velx, tb->velx, (size_t)tb->s.nxp);
+    if (st == KTM_OK) st = upload(&p->d_vely, tb->vely, (size_t)tb->s.nyp);
+    if (st == KTM_OK) st = upload(&p->d_slab_cx, tb->slab_cx, (size_t)tb->s.no_job);
+    if (st == KTM_OK) st = upload(&p->d_slab_cy, tb->slab_cy, (size_t)tb->s.no_job);
+    if (st == KTM_OK) st = upload(&p->d_slab_thr, tb->slab_thr, (size_t)tb->s.no_job);
+    if (st == KTM_OK) st = upload(&p->d_w, tb->w, (size_t)ns + 2);
+    if (st == KTM_OK) st = upload(&p->d_hconv, tb->hconv, (size_t)ns);
+    if (st != KTM_OK) { ktm_plan_destroy(p); return st; }
+    *out = p;
+    return KTM_OK;
+}
+
+extern "C" int ktm_plan_setup(const ktm_plan *p, ktm_setup *out)
+{
+    if (!p || !out) return KTM_ERR_ARG;
+    *out = p->tb.s;
+    return KTM_OK;
+}
+
+extern "C" int64_t ktm_plan_trace_row(const ktm_plan *p) { return p ? p->row : 0; }
+
+extern "C" int64_t ktm_plan_image_size(const ktm_plan *p)
+{
+    if (!p) return 0;
+    return (int64_t)p->tb.s.nxp * p->tb.s.nyp * p->tb.s.nh * p->tb.s.ns;
+}
+
+static GeoArgs geo_args(const ktm_plan *p)
+{
+    GeoArgs g;
+    g.ox = p->cfg.ox; g.oy = p->cfg.oy; g.scale = p->cfg.scale_xy;
+    g.oh = p->cfg.oh; g.dh = p->cfg.dh;
+    g.sinA = p->tb.s.sinA; g.cosA = p->tb.s.cosA;
+    return g;
+}
+
+extern "C" int ktm_geometry(const ktm_plan *p, const int32_t *coords, int64_t ntr,
+                            ktm_trace_geom *geom, void *stream)
+{
+    if (!p || ntr < 0 || (ntr && (!coords || !geom))) return KTM_ERR_ARG;
+    if (!ntr) return KTM_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const int nt = 256;
+    hipLaunchKernelGGL(k_geometry, dim3((unsigned)((ntr + nt - 1) / nt)), dim3(nt), 0, s,
+                       geo_args(p), coords, ntr, geom);
+    HIP_TRY(hipGetLastError());
+    return KTM_OK;
+}
+
+extern "C" int ktm_prep(const ktm_plan *p, const float *traces, const int32_t *coords,
+                        int64_t ntr, int prep_mode, int dtype, void *prepared,
+                        ktm_trace_geom *geom, void *stream)
+{
+    if (!p || ntr < 0) return KTM_ERR_ARG;
+    if (!ntr) return KTM_OK;
+    if (!traces || !coords || !prepared || !geom) return KTM_ERR_ARG;
+    if (prep_mode != KTM_PREP_REFERENCE && prep_mode != KTM_PREP_CLOSED) return KTM_ERR_ARG;
+    if (dtype != KTM_TRACE_F32 && dtype != KTM_TRACE_BF16 && dtype != KTM_TRACE_F16)
+        return KTM_ERR_ARG;
+    /* reduced-precision storage keeps the raw trace; its filter is the
+     * closed form evaluated while staging */
+    if (dtype != KTM_TRACE_F32 && prep_mode != KTM_PREP_CLOSED) return KTM_ERR_UNSUPPORTED;
+    hipStream_t s = (hipStream_t)stream;
+    int st = ktm_geometry(p, coords, ntr, geom, stream);
+    if (st != KTM_OK) return st;
+    const int ns = p->tb.s.ns;
+    if (dtype == KTM_TRACE_F32) {
+        if (prep_mode == KTM_PREP_REFERENCE) {
+            const int nt = 64;
+            hipLaunchKernelGGL(k_prep_ref, dim3((unsigned)((ntr + nt - 1) / nt)), dim3(nt), 0, s,
+                               traces, ntr, ns, p->row, p->tb.s.dt, p->d_t2, (float *)prepared);
+        } else {
+            if (ntr > 65535 * 1024LL) return KTM_ERR_UNSUPPORTED;
+            const int nt = 256;
+            for (int64_t i0 = 0; i0 < ntr; i0 += 65535) {
+                const int64_t n = ntr - i0 < 65535 ? ntr - i0 : 65535;
+                hipLaunchKernelGGL(k_prep_closed, dim3((unsigned)((p->row + nt - 1) / nt), (unsigned)n),
+                                   dim3(nt), 0, s, traces + i0 * ns, n, ns, p->row, p->d_w,
+                                   (float *)prepared + i0 * p->row);
+            }
+        }
+    } else {
+        const int nt = 256, tpb = nt / KTM_WAVE;
+        const dim3 grid((unsigned)((ntr + tpb - 1) / tpb));
+        if (dtype == KTM_TRACE_BF16)
+            hipLaunchKernelGGL(k_quantize<KTM_TRACE_BF16>, grid, dim3(nt), 0, s, traces, ntr, ns,
+                               p->row, (uint16_t *)prepared, geom);
+        else
+            hipLaunchKernelGGL(k_quantize<KTM_TRACE_F16>, grid, dim3(nt), 0, s, traces, ntr, ns,
+                               p->row, (uint16_t *)prepared, geom);
+    }
+    HIP_TRY(hipGetLastError());
+    return KTM_OK;
+}
+
+extern "C" int ktm_image_reset(const ktm_plan *p, float *km, float *fold, float *comp, void *stream)
+{
+    if (!p || !km || !fold) return KTM_ERR_ARG;
+    const int64_t n = ktm_plan_image_size(p);
+    hipLaunchKernelGGL(k_image_reset, dim3(2048), dim3(256), 0, (hipStream_t)stream, km, fold, comp, n);
+    HIP_TRY(hipGetLastError());
+    return KTM_OK;
+}
+
+/* kernel choice: K sample slots per lane and W waves per column cover ns */
+typedef void (*mig_fn)(const MigArgs);
+
+template <int K, int W>
+static mig_fn pick_src(int dtype, int accum)
+{
+    if (dtype == KTM_TRACE_F32)
+        return accum == KTM_ACCUM_KAHAN ? k_migrate<K, W, 8, true, KTM_TRACE_F32>
+                                        : k_migrate<K, W, 8, false, KTM_TRACE_F32>;
+    if (dtype == KTM_TRACE_BF16) return k_migrate<K, W, 8, true, KTM_TRACE_BF16>;
+    return k_migrate<K, W, 8, true, KTM_TRACE_F16>;
+}
+
+/* W waves share one image column (each owns K*64 consecutive samples);
+ * a workgroup is 8 waves, so it covers 8/W columns. */
+static mig_fn pick_migrate(int ns, int dtype, int accum, int *W)
+{
+    if (ns <= 512) { *W = 1; return pick_src<8, 1>(dtype, accum); }
+    if (ns <= 1024) { *W = 2; return pick_src<8, 2>(dtype, accum); }
+    if (ns <= 1536) { *W = 2; return pick_src<12, 2>(dtype, accum); }
+    if (ns <= 2048) { *W = 4; return pick_src<8, 4>(dtype, accum); }
+    *W = 4;
+    return pick_src<12, 4>(dtype, accum);
+}
+
+extern "C" int ktm_migrate_block(const ktm_plan *p, const void *prepared, int dtype, int prep_mode,
+                                 const ktm_trace_geom *geom, int64_t ntr, int accum, int job,
+                                 float *km, float *fold, float *comp, uint64_t *counter,
+                                 void *stream)
+{
+    if (!p || !km || !fold || ntr < 0) return KTM_ERR_ARG;
+    if (!ntr) return KTM_OK;
+    if (!prepared || !geom) return KTM_ERR_ARG;
+    if (accum != KTM_ACCUM_PLAIN && accum != KTM_ACCUM_KAHAN) return KTM_ERR_ARG;
+    if (accum == KTM_ACCUM_KAHAN && !comp) return KTM_ERR_ARG;
+    if (dtype != KTM_TRACE_F32 && (accum != KTM_ACCUM_KAHAN || prep_mode != KTM_PREP_CLOSED))
+        return KTM_ERR_UNSUPPORTED;
+    (void)prep_mode;
+    const ktm_setup *s = &p->tb.s;
+    MigArgs a;
+    memset(&a, 0, sizeof a);
+    a.prep = prepared; a.geom = geom; a.ntr = ntr; a.row = p->row;
+    a.km = km; a.fold = fold; a.comp = comp;
+    a.vel2 = p->d_vel2; a.t1 = p->d_t1; a.velx = p->d_velx; a.vely = p->d_vely;
+    a.t2 = p->d_t2; a.colx = p->d_colx; a.coly = p->d_coly;
+    a.slab_cx = p->d_slab_cx; a.slab_cy = p->d_slab_cy; a.slab_thr = p->d_slab_thr; a.w = p->d_w;
+    a.counter = (unsigned long long *)counter;
+    a.aperture = p->cfg.aperture;
+    a.dt = s->dt;
+    a.rdt = 1.0f / s->dt;
+    a.ns = s->ns; a.nh = s->nh; a.nxp = s->nxp; a.nyp = s->nyp; a.nx = s->nx; a.ny = s->ny;
+    a.jobs_y = p->cfg.jobs_y; a.nvel_xl = p->cfg.nvel_xlines; a.nsm = s->nsm;
+    if (job >= s->no_job) return KTM_ERR_ARG;
+    if (job < 0) {
+        a.rx0 = 0; a.ry0 = 0; a.rx1 = s->nxp; a.ry1 = s->nyp;
+    } else { /* slab of Kirchhoff.c:1341-1347, job -> slice as in 287-295 */
+        a.rx0 = (job / p->cfg.jobs_y) * s->nx; a.rx1 = a.rx0 + s->nx;
+        a.ry0 = (job % p->cfg.jobs_y) * s->ny; a.ry1 = a.ry0 + s->ny;
+    }
+    int W = 1;
+    mig_fn fn = pick_migrate(s->ns, dtype, accum, &W);
+    /* tile of 8/W columns: a row on a 2-D line, as square as possible in 3-D */
+    const int ncol = 8 / W;
+    a.tx_cols = s->nyp == 1 ? ncol : (ncol == 8 ? 4 : ncol == 4 ? 2 : ncol);
+    a.ty_cols = ncol / a.tx_cols;
+    a.tiles_x = (a.rx1 - a.rx0 + a.tx_cols - 1) / a.tx_cols;
+    const int tiles_y = (a.ry1 - a.ry0 + a.ty_cols - 1) / a.ty_cols;
+    a.ntiles = a.tiles_x * tiles_y;
+    const int nthreads = 8 * KTM_WAVE;
+    const size_t lds = sizeof(float) * (size_t)(8 * p->row + 8 * 8) + sizeof(int) * (size_t)(nthreads + 8) +
+                       (dtype != KTM_TRACE_F32 ? sizeof(float) * (size_t)(s->ns + 2) : 0);
+    if (lds > 160 * 1024) return KTM_ERR_UNSUPPORTED;
+    const int per = (a.ntiles + 7) / 8;
+    const dim3 grid((unsigned)(per * 8), (unsigned)s->nh);
+    HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(fn, grid, dim3(nthreads), lds, (hipStream_t)stream, a);
+    HIP_TRY(hipGetLastError());
+    return KTM_OK;
+}
+
+extern "C" int ktm_poststack(const ktm_plan *p, const float *km, const float *fold, float *out,
+                             void *stream)
+{
+    if (!p || !km || !fold || !out) return KTM_ERR_ARG;
+    const ktm_setup *s = &p->tb.s;
+    const int slots = (s->ns + PS_NT - 1) / PS_NT;
+    typedef void (*ps_fn)(const float *, const float *, int64_t, int, float, const double *, float *);
+    ps_fn fn = slots <= 4 ? k_poststack<4> : slots <= 8 ? k_poststack<8> : k_poststack<12>;
+    if (slots > 12) return KTM_ERR_UNSUPPORTED;
+    const int64_t ntraces = (int64_t)s->nxp * s->nyp * s->nh;
+    const size_t lds = sizeof(double) * (size_t)(2 * s->ns + s->ns * PS_TR);
+    const int64_t nblk = (ntraces + PS_TR - 1) / PS_TR;
+    HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(fn, dim3((unsigned)nblk), dim3(PS_NT), lds, (hipStream_t)stream,
+                       km, fold, ntraces, s->ns, s->dt, p->d_hconv, out);
+    HIP_TRY(hipGetLastError());
+    return KTM_OK;
+}
+
+extern "C" int ktm_device_alloc(void **ptr, size_t bytes)
+{
+    if (!ptr) return KTM_ERR_ARG;
+    *ptr = nullptr;
+    return hipMalloc(ptr, bytes ? bytes : 1) == hipSuccess ? KTM_OK : KTM_ERR_NOMEM;
+}
+
+extern "C" int ktm_device_free(void *ptr)
+{
+    return hipFree(ptr) == hipSuccess ? KTM_OK : KTM_ERR_DEVICE;
+}
+
+extern "C" int ktm_copy_to_device(void *dst, const void *src, size_t bytes)
+{
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? KTM_OK : KTM_ERR_DEVICE;
+}
+
+extern "C" int ktm_copy_to_host(void *dst, const void *src, size_t bytes)
+{
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? KTM_OK : KTM_ERR_DEVICE;
+}
+
+extern "C" int ktm_device_sync(void)
+{
+    return hipDeviceSynchronize() == hipSuccess ? KTM_OK : KTM_ERR_DEVICE;
+}

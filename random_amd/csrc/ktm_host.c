@@ -1,0 +1,506 @@
+/*
+ * ktm_host.c -- C host layer: SEG-Y input/output, the whole-survey driver
+ * ktm_migrate(), and the reference-signature shim kirchhoff(job_id).
+ *
+ * The reference reads and writes SEG-Y itself inside kirchhoff()
+ * (Kirchhoff.c:1121-1136, 1259-1290, 1414-1593) and gets its setup from
+ * main() through globals (Kirchhoff.c:103-115, 215-373).  Here the file
+ * handling stays on the host in C, in streamed blocks of ntblock traces, and
+ * the migration itself goes to the device through the C ABI of ktm.h.
+ */
+#define _FILE_OFFSET_BITS 64
+#define _POSIX_C_SOURCE 200809L
+#include "ktm_internal.h"
+
+#include <errno.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <sys/types.h>
+#include <time.h>
+
+#define SEGY_FILE_HDR 3600
+#define SEGY_TRACE_HDR 240
+
+/* ------------------------------------------------------------------------ */
+/* IBM hex float -> IEEE, Kirchhoff.c:882-895 + 1284-1290                   */
+/* ------------------------------------------------------------------------ */
+static float ibm_word(uint32_t w)
+{
+    /* value = sign * (fraction / 2^24) * (float)16^(exponent-64), evaluated
+     * as the reference does: the power of 16 is rounded to float on its own
+     * (overflowing to inf above 2^127, flushing to 0 below 2^-149) before the
+     * single float product, so a zero fraction with a huge exponent is NaN */
+    const int32_t iw = (int32_t)w;
+    const int sign = ((iw >> 31) & 1) ? -1 : 1;
+    const float frac = (float)(iw & 0x00ffffff) / 16777216.0f;
+    const int e4 = 4 * (((iw >> 24) & 0x7f) - 64);
+    const float p16 = (float)ldexp(1.0, e4);
+    return (float)sign * frac * p16;
+}
+
+void ktm_ibm_to_float(const uint8_t *bytes, int64_t nwords, float *out)
+{
+    for (int64_t i = 0; i < nwords; ++i) out[i] = ibm_word(ktm_get_be32(bytes + 4 * i));
+}
+
+static float ieee_word(uint32_t w)
+{
+    float f;
+    memcpy(&f, &w, 4);
+    return f;
+}
+
+/* ------------------------------------------------------------------------ */
+/* SEG-Y reading, READ_SEGY.c:169-209 / Kirchhoff.c:633-722, 1259-1297       */
+/* ------------------------------------------------------------------------ */
+static int file_size(FILE *f, int64_t *len)
+{
+    if (fseeko(f, 0, SEEK_END) != 0) return KTM_ERR_IO;
+    const off_t n = ftello(f);
+    if (n < 0) return KTM_ERR_IO;
+    *len = (int64_t)n;
+    return KTM_OK;
+}
+
+int ktm_segy_count(const char *path, int *ns, float *dt, int *nt)
+{
+    if (!path || !ns || !dt || !nt) return KTM_ERR_ARG;
+    FILE *f = fopen(path, "rb");
+    if (!f) return KTM_ERR_IO;
+    int64_t len = 0;
+    uint8_t b[4];
+    int st = file_size(f, &len);
+    if (st == KTM_OK && len < SEGY_FILE_HDR + SEGY_TRACE_HDR) st = KTM_ERR_FORMAT;
+    /* ns, dt at bytes 115-118 of the first trace header (Kirchhoff.c:652-658) */
+    if (st == KTM_OK && (fseeko(f, SEGY_FILE_HDR + 114, SEEK_SET) != 0 || fread(b, 1, 4, f) != 4))
+        st = KTM_ERR_IO;
+    fclose(f);
+    if (st != KTM_OK) return st;
+    const uint16_t ns1 = ktm_get_be16(b), dt1 = ktm_get_be16(b + 2);
+    if (ns1 == 0) return KTM_ERR_FORMAT;
+    *ns = (int)ns1;
+    *dt = (float)(dt1 / 1000000.0);
+    *nt = (int)((len - SEGY_FILE_HDR) / (SEGY_TRACE_HDR + 4 * (int64_t)ns1)); /* :659 */
+    return KTM_OK;
+}
+
+int ktm_segy_read(const char *path, int64_t first, int64_t count, int ieee, float *samples,
+                  int32_t *coords, int32_t *extra)
+{
+    int ns, nt;
+    float dt;
+    int st = ktm_segy_count(path, &ns, &dt, &nt);
+    if (st != KTM_OK) return st;
+    if (first < 0 || count < 0 || first + count > nt) return KTM_ERR_ARG;
+    FILE *f = fopen(path, "rb");
+    if (!f) return KTM_ERR_IO;
+    const int64_t rec = SEGY_TRACE_HDR + 4 * (int64_t)ns;
+    /* trace n starts at 3600 + (240 + 4 ns) (n-1), READ_SEGY.c:227,441 */
+    if (fseeko(f, SEGY_FILE_HDR + first * rec, SEEK_SET) != 0) { fclose(f); return KTM_ERR_IO; }
+    const int64_t chunk = 4096;
+    uint8_t *buf = (uint8_t *)malloc((size_t)(rec * chunk));
+    if (!buf) { fclose(f); return KTM_ERR_NOMEM; }
+    for (int64_t i0 = 0; i0 < count && st == KTM_OK; i0 += chunk) {
+        const int64_t n = count - i0 < chunk ? count - i0 : chunk;
+        if (fread(buf, (size_t)rec, (size_t)n, f) != (size_t)n) { st = KTM_ERR_IO; break; }
+        for (int64_t k = 0; k < n; ++k) {
+            const uint8_t *h = buf + k * rec;
+            const int64_t i = i0 + k;
+            if (coords) /* header words 18-21: sx, sy, gx, gy (Kirchhoff.c:1297,1302) */
+                for (int c = 0; c < 4; ++c) coords[4 * i + c] = (int32_t)ktm_get_be32(h + 72 + 4 * c);
+            if (extra) { /* words 47-48: iline, xline (Kirchhoff.c:537-538) */
+                extra[2 * i] = (int32_t)ktm_get_be32(h + 188);
+                extra[2 * i + 1] = (int32_t)ktm_get_be32(h + 192);
+            }
+            if (samples) {
+                float *o = samples + i * ns;
+                const uint8_t *d = h + SEGY_TRACE_HDR;
+                if (ieee)
+                    for (int t = 0; t < ns; ++t) o[t] = ieee_word(ktm_get_be32(d + 4 * t));
+                else
+                    ktm_ibm_to_float(d, ns, o);
+            }
+        }
+    }
+    free(buf);
+    fclose(f);
+    return st;
+}
+
+int ktm_segy_write(const char *path, const uint8_t *file_header3600, const uint8_t *trace_headers,
+                   const float *samples, int64_t ntr, int ns, int ieee)
+{
+    if (!path || !file_header3600 || (ntr > 0 && (!trace_headers || !samples)) || ns <= 0)
+        return KTM_ERR_ARG;
+    FILE *f = fopen(path, "wb");
+    if (!f) return KTM_ERR_IO;
+    int st = fwrite(file_header3600, 1, SEGY_FILE_HDR, f) == SEGY_FILE_HDR ? KTM_OK : KTM_ERR_IO;
+    uint8_t *d = (uint8_t *)malloc((size_t)ns * 4);
+    if (!d) st = KTM_ERR_NOMEM;
+    for (int64_t i = 0; i < ntr && st == KTM_OK; ++i) {
+        const float *s = samples + i * ns;
+        for (int t = 0; t < ns; ++t) {
+            uint32_t u;
+            if (ieee) {
+                memcpy(&u, s + t, 4);
+            } else { /* IEEE -> IBM (truncating), for writing IBM test files */
+                const float v = s[t];
+                u = 0;
+                if (v != 0.0f && isfinite(v)) {
+                    int e2;
+                    double m = frexp(fabs((double)v), &e2); /* v = m 2^e2, m in [0.5,1) */
+                    int e16 = (e2 + 3) >> 2;                /* ceil(e2/4) */
+                    double frac = m * ldexp(1.0, e2 - 4 * e16);
+                    uint32_t mant = (uint32_t)(frac * 16777216.0);
+                    if (mant > 0xffffff) { mant >>= 4; ++e16; }
+                    u = ((v < 0.0f) ? 0x80000000u : 0u) | ((uint32_t)(e16 + 64) << 24) | mant;
+                }
+            }
+            ktm_put_be32(d + 4 * t, u);
+        }
+        if (fwrite(trace_headers + i * SEGY_TRACE_HDR, 1, SEGY_TRACE_HDR, f) != SEGY_TRACE_HDR ||
+            fwrite(d, 4, (size_t)ns, f) != (size_t)ns)
+            st = KTM_ERR_IO;
+    }
+    free(d);
+    if (fclose(f) != 0 && st == KTM_OK) st = KTM_ERR_IO;
+    return st;
+}
+
+int ktm_segy_read_velocity(const char *path, const ktm_config *cfg, float *vel, int *nsm,
+                           float *dtm, int64_t *nzero)
+{
+    /* build_vel_model(), Kirchhoff.c:476-546: grid index by integer division
+     * of iline/xline (words 47/48), out-of-grid traces ignored */
+    int ns1, nt1;
+    float dt1;
+    int st = ktm_segy_count(path, &ns1, &dt1, &nt1);
+    if (st != KTM_OK) return st;
+    if (nsm) *nsm = ns1;
+    if (dtm) *dtm = dt1;
+    if (!vel) return KTM_OK;
+    const int nil = cfg->nvel_ilines, nxl = cfg->nvel_xlines;
+    memset(vel, 0, sizeof(float) * (size_t)nil * nxl * ns1);
+    const int64_t block = 5000; /* ntblock_vel, Kirchhoff.c:503 */
+    float *s = (float *)malloc(sizeof(float) * (size_t)(block * ns1));
+    int32_t *ex = (int32_t *)malloc(sizeof(int32_t) * (size_t)(2 * block));
+    if (!s || !ex) { free(s); free(ex); return KTM_ERR_NOMEM; }
+    for (int64_t i0 = 0; i0 < nt1 && st == KTM_OK; i0 += block) {
+        const int64_t n = nt1 - i0 < block ? nt1 - i0 : block;
+        st = ktm_segy_read(path, i0, n, 0, s, NULL, ex);
+        for (int64_t k = 0; k < n && st == KTM_OK; ++k) {
+            const int i = (ex[2 * k] - cfg->ovel_ilines) / cfg->dvel_ilines;
+            const int j = (ex[2 * k + 1] - cfg->ovel_xlines) / cfg->dvel_xlines;
+            if (i >= 0 && i < nil && j >= 0 && j < nxl)
+                memcpy(vel + ((size_t)i * nxl + j) * ns1, s + k * ns1, sizeof(float) * (size_t)ns1);
+        }
+    }
+    free(s);
+    free(ex);
+    if (st == KTM_OK && nzero) { /* the zero report of Kirchhoff.c:614-615 */
+        int64_t z = 0;
+        for (size_t k = 0; k < (size_t)nil * nxl * ns1; ++k) z += vel[k] == 0.0f;
+        *nzero = z;
+    }
+    return st;
+}
+
+/* ------------------------------------------------------------------------ */
+/* SEG-Y headers of the image, Kirchhoff.c:729-814 and 1437-1590            */
+/* ------------------------------------------------------------------------ */
+void ktm_segy_file_header(const ktm_config *cfg, int ns, float dt, int fix_count, uint8_t out[3600])
+{
+    ktm_setup s;
+    memset(out, 0, SEGY_FILE_HDR);
+    if (ktm_setup_init(cfg, ns, dt, 1, dt, &s, NULL) != KTM_OK && s.Nx == 0) return;
+    /* the second sprintf of Kirchhoff.c:757-758 overwrites the first */
+    static const char text[] = "written by G. O'Brien\n";
+    memcpy(out, text, sizeof text - 1);
+    uint8_t *bin = out + 3200;
+    if (fix_count) ktm_put_be16(bin + 12, (uint16_t)(s.Nx * s.Ny * cfg->nh));
+    ktm_put_be16(bin + 16, (uint16_t)(dt * 1000000)); /* htons(dt*1000000), :786 */
+    ktm_put_be16(bin + 20, (uint16_t)ns);
+    ktm_put_be16(bin + 22, (uint16_t)ns);
+    ktm_put_be16(bin + 24, 5);                       /* IEEE format, :797-799 */
+    ktm_put_be16(bin + 26, 1);
+    ktm_put_be16(bin + 28, 1);
+    ktm_put_be16(bin + 30, 1);
+}
+
+void ktm_segy_image_trace_header(const ktm_config *cfg, int ns, float dt, int job, int xc, int yc,
+                                 int ih, uint8_t out[240])
+{
+    ktm_setup s;
+    memset(out, 0, SEGY_TRACE_HDR);
+    ktm_setup_init(cfg, ns, dt, 1, dt, &s, NULL);
+    const int X = xc + (job / cfg->jobs_y) * s.nx;
+    const int Y = yc + (job % cfg->jobs_y) * s.ny;
+    const double il = cfg->start_iline + X * cfg->inc_iline; /* Kirchhoff.c:1482 */
+    const double xl = cfg->start_xline + Y * cfg->inc_xline; /* Kirchhoff.c:1483 */
+    const int xm = (int)(il * cfg->iline_distance);          /* Kirchhoff.c:1485 (int xc1) */
+    const int ym = (int)(xl * cfg->xline_distance);          /* Kirchhoff.c:1486 (int yc1) */
+    /* running trace number (Kirchhoff.c:1438,1594) */
+    const int tracl = s.nx * s.ny * cfg->nh * job + 1 + (xc * s.ny + yc) * cfg->nh + ih;
+    ktm_put_be32(out + 0, (uint32_t)tracl);
+    ktm_put_be32(out + 20, (uint32_t)(X + s.Nx * Y + 1));                  /* cdp, :1444 */
+    ktm_put_be32(out + 36, (uint32_t)(int)ceil(cfg->oh + ih * cfg->dh));   /* offset, :1484 */
+    for (int c = 0; c < 4; ++c) ktm_put_be32(out + 72 + 4 * c, 1u);        /* sx..gy = 1 */
+    ktm_put_be16(out + 114, (uint16_t)ns);
+    ktm_put_be16(out + 116, (uint16_t)(dt * 1000000));
+    /* cdpx/cdpy, Kirchhoff.c:1492-1493: (int)ox + float terms, truncated */
+    const float cx = (int)cfg->ox + s.cosA * xm - s.sinA * ym;
+    const float cy = (int)cfg->oy + s.sinA * xm + s.cosA * ym;
+    ktm_put_be32(out + 180, (uint32_t)(int)cx);
+    ktm_put_be32(out + 184, (uint32_t)(int)cy);
+    ktm_put_be32(out + 188, (uint32_t)(int)il);
+    ktm_put_be32(out + 192, (uint32_t)(int)xl);
+    ktm_put_be32(out + 236, 9999u); /* Kirchhoff.c:1589-1590 */
+}
+
+/* ------------------------------------------------------------------------ */
+/* whole-survey driver                                                       */
+/* ------------------------------------------------------------------------ */
+static double now_ms(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
+typedef struct dev_bufs {
+    float *traces, *prepared, *km, *fold, *image;
+    int32_t *coords;
+    ktm_trace_geom *geom;
+    uint64_t *counter;
+} dev_bufs;
+
+static void free_bufs(dev_bufs *b)
+{
+    ktm_device_free(b->traces); ktm_device_free(b->prepared); ktm_device_free(b->km);
+    ktm_device_free(b->fold); ktm_device_free(b->image); ktm_device_free(b->coords);
+    ktm_device_free(b->geom); ktm_device_free(b->counter);
+    memset(b, 0, sizeof *b);
+}
+
+/* One pass over all traces in blocks of ntblock (Kirchhoff.c:1276-1408),
+ * for every job (job < 0) or one job.  Traces come from the host arrays
+ * (traces/coords) or, when path != NULL, from the SEG-Y file. */
+static int run_survey(const ktm_plan *plan, const ktm_config *cfg, const char *path,
+                      const float *traces, const int32_t *coords, int64_t ntr, int job,
+                      dev_bufs *b, uint64_t *contrib, double *mig_ms)
+{
+    ktm_setup s;
+    ktm_plan_setup(plan, &s);
+    const int64_t blk = cfg->ntblock > 0 ? cfg->ntblock : 50000;
+    const int64_t row = ktm_plan_trace_row(plan);
+    const int64_t nb = ntr < blk ? ntr : blk;
+    int st = KTM_OK;
+    float *htr = NULL;
+    int32_t *hco = NULL;
+    if (path) {
+        htr = (float *)malloc(sizeof(float) * (size_t)(nb * s.ns));
+        hco = (int32_t *)malloc(sizeof(int32_t) * (size_t)(4 * nb));
+        if (!htr || !hco) st = KTM_ERR_NOMEM;
+    }
+    if (st == KTM_OK) st = ktm_device_alloc((void **)&b->traces, sizeof(float) * (size_t)(nb * s.ns));
+    if (st == KTM_OK) st = ktm_device_alloc((void **)&b->coords, sizeof(int32_t) * (size_t)(4 * nb));
+    if (st == KTM_OK) st = ktm_device_alloc((void **)&b->prepared, sizeof(float) * (size_t)(nb * row));
+    if (st == KTM_OK) st = ktm_device_alloc((void **)&b->geom, sizeof(ktm_trace_geom) * (size_t)nb);
+    if (st == KTM_OK) st = ktm_device_alloc((void **)&b->counter, sizeof(uint64_t));
+    uint64_t zero = 0;
+    if (st == KTM_OK) st = ktm_copy_to_device(b->counter, &zero, sizeof zero);
+    double t_mig = 0.0;
+    for (int64_t i0 = 0; i0 < ntr && st == KTM_OK; i0 += blk) {
+        const int64_t n = ntr - i0 < blk ? ntr - i0 : blk;
+        const float *src_tr = traces + i0 * s.ns;
+        const int32_t *src_co = coords + 4 * i0;
+        if (path) {
+            st = ktm_segy_read(path, i0, n, 0, htr, hco, NULL);
+            src_tr = htr;
+            src_co = hco;
+        }
+        for (int64_t k = 0; k < 4 * n && st == KTM_OK; ++k)
+            if (src_co[k] < 0) st = KTM_ERR_ARG; /* htonl() - ox on a negative word is undefined */
+        if (st == KTM_OK) st = ktm_copy_to_device(b->traces, src_tr, sizeof(float) * (size_t)(n * s.ns));
+        if (st == KTM_OK) st = ktm_copy_to_device(b->coords, src_co, sizeof(int32_t) * (size_t)(4 * n));
+        if (st == KTM_OK)
+            st = ktm_prep(plan, b->traces, b->coords, n, KTM_PREP_REFERENCE, KTM_TRACE_F32,
+                          b->prepared, b->geom, NULL);
+        const double t0 = now_ms();
+        if (st == KTM_OK)
+            st = ktm_migrate_block(plan, b->prepared, KTM_TRACE_F32, KTM_PREP_REFERENCE, b->geom, n,
+                                   KTM_ACCUM_PLAIN, job, b->km, b->fold, NULL, b->counter, NULL);
+        if (st == KTM_OK) st = ktm_device_sync();
+        t_mig += now_ms() - t0;
+    }
+    if (st == KTM_OK) st = ktm_copy_to_host(contrib, b->counter, sizeof(uint64_t));
+    *mig_ms = t_mig;
+    free(htr);
+    free(hco);
+    return st;
+}
+
+int ktm_migrate(const ktm_config *cfg, int ns, float dt, const float *traces, const int32_t *coords,
+                int64_t ntr, const float *vel, int nsm, float dtm, float *km, float *fold,
+                float *image, ktm_stats *stats)
+{
+    if (!cfg || ntr < 0 || (ntr > 0 && (!traces || !coords)) || !vel) return KTM_ERR_ARG;
+    const double t_start = now_ms();
+    ktm_plan *plan = NULL;
+    int st = ktm_plan_create(&plan, cfg, ns, dt, nsm, dtm, vel);
+    if (st != KTM_OK) return st;
+    const size_t nimg = (size_t)ktm_plan_image_size(plan);
+    dev_bufs b;
+    memset(&b, 0, sizeof b);
+    st = ktm_device_alloc((void **)&b.km, sizeof(float) * nimg);
+    if (st == KTM_OK) st = ktm_device_alloc((void **)&b.fold, sizeof(float) * nimg);
+    if (st == KTM_OK) st = ktm_image_reset(plan, b.km, b.fold, NULL, NULL);
+    uint64_t contrib = 0;
+    double mig_ms = 0.0;
+    if (st == KTM_OK)
+        st = run_survey(plan, cfg, NULL, traces, coords, ntr, -1, &b, &contrib, &mig_ms);
+    if (st == KTM_OK && image) {
+        st = ktm_device_alloc((void **)&b.image, sizeof(float) * nimg);
+        if (st == KTM_OK) st = ktm_poststack(plan, b.km, b.fold, b.image, NULL);
+        if (st == KTM_OK) st = ktm_copy_to_host(image, b.image, sizeof(float) * nimg);
+    }
+    if (st == KTM_OK && km) st = ktm_copy_to_host(km, b.km, sizeof(float) * nimg);
+    if (st == KTM_OK && fold) st = ktm_copy_to_host(fold, b.fold, sizeof(float) * nimg);
+    free_bufs(&b);
+    ktm_plan_destroy(plan);
+    if (stats) {
+        stats->contributions = contrib;
+        stats->migrate_ms = mig_ms;
+        stats->total_ms = now_ms() - t_start;
+    }
+    return st;
+}
+
+/* ------------------------------------------------------------------------ */
+/* kirchhoff(job_id): the reference signature                                */
+/* ------------------------------------------------------------------------ */
+static struct {
+    int ready, status;
+    ktm_config cfg;
+    char input_data[4096], image_dir[4096];
+    int ns, nt;
+    float dt;
+    ktm_plan *plan;
+} G;
+
+int ktm_last_status(void) { return G.status; }
+
+void ktm_globals_free(void)
+{
+    ktm_plan_destroy(G.plan);
+    memset(&G, 0, sizeof G);
+}
+
+int ktm_globals_init(const ktm_config *cfg, const char *input_data, const char *input_vel,
+                     const float *vel, int nsm, float dtm, const char *image_dir)
+{
+    ktm_globals_free();
+    if (!cfg || !input_data || !image_dir || (!input_vel && !vel)) return G.status = KTM_ERR_ARG;
+    if (strlen(input_data) >= sizeof G.input_data || strlen(image_dir) >= sizeof G.image_dir)
+        return G.status = KTM_ERR_ARG;
+    G.cfg = *cfg;
+    strcpy(G.input_data, input_data);
+    strcpy(G.image_dir, image_dir);
+    /* count_size_file(), Kirchhoff.c:263 */
+    int st = ktm_segy_count(input_data, &G.ns, &G.dt, &G.nt);
+    if (st != KTM_OK) return G.status = st;
+    /* mkdir image_dir (Kirchhoff.c:254-260); an existing directory is fine */
+    if (mkdir(image_dir, 0777) != 0 && errno != EEXIST) return G.status = KTM_ERR_IO;
+    /* build_vel_model(), Kirchhoff.c:328 */
+    float *vbuf = NULL;
+    if (input_vel) {
+        st = ktm_segy_read_velocity(input_vel, cfg, NULL, &nsm, &dtm, NULL);
+        if (st != KTM_OK) return G.status = st;
+        vbuf = (float *)malloc(sizeof(float) * (size_t)cfg->nvel_ilines * cfg->nvel_xlines * nsm);
+        if (!vbuf) return G.status = KTM_ERR_NOMEM;
+        st = ktm_segy_read_velocity(input_vel, cfg, vbuf, &nsm, &dtm, NULL);
+        if (st != KTM_OK) { free(vbuf); return G.status = st; }
+        vel = vbuf;
+    }
+    st = ktm_plan_create(&G.plan, cfg, G.ns, G.dt, nsm, dtm, vel);
+    free(vbuf);
+    if (st != KTM_OK) return G.status = st;
+    /* ebdic_header.bin / binary_header.bin, Kirchhoff.c:321-322, 729-814 */
+    uint8_t hdr[3600];
+    ktm_segy_file_header(cfg, G.ns, G.dt, 0, hdr);
+    char name[4200];
+    snprintf(name, sizeof name, "%s/ebdic_header.bin", image_dir);
+    FILE *f = fopen(name, "wb");
+    if (!f) return G.status = KTM_ERR_IO;
+    int ok = fwrite(hdr, 1, 3200, f) == 3200;
+    ok &= fclose(f) == 0;
+    snprintf(name, sizeof name, "%s/binary_header.bin", image_dir);
+    f = fopen(name, "wb");
+    if (!f) return G.status = KTM_ERR_IO;
+    ok &= fwrite(hdr + 3200, 1, 400, f) == 400;
+    ok &= fclose(f) == 0;
+    if (!ok) return G.status = KTM_ERR_IO;
+    G.ready = 1;
+    return G.status = KTM_OK;
+}
+
+static int kirchhoff_job(int job_id)
+{
+    if (!G.ready) return KTM_ERR_ARG;
+    ktm_setup s;
+    ktm_plan_setup(G.plan, &s);
+    if (job_id < 0 || job_id >= s.no_job) return KTM_ERR_ARG;
+    const size_t nimg = (size_t)ktm_plan_image_size(G.plan);
+    dev_bufs b;
+    memset(&b, 0, sizeof b);
+    int st = ktm_device_alloc((void **)&b.km, sizeof(float) * nimg);
+    if (st == KTM_OK) st = ktm_device_alloc((void **)&b.fold, sizeof(float) * nimg);
+    if (st == KTM_OK) st = ktm_device_alloc((void **)&b.image, sizeof(float) * nimg);
+    if (st == KTM_OK) st = ktm_image_reset(G.plan, b.km, b.fold, NULL, NULL);
+    uint64_t contrib = 0;
+    double ms = 0.0;
+    if (st == KTM_OK)
+        st = run_survey(G.plan, &G.cfg, G.input_data, NULL, NULL, G.nt, job_id, &b, &contrib, &ms);
+    if (st == KTM_OK) st = ktm_poststack(G.plan, b.km, b.fold, b.image, NULL);
+    float *img = NULL;
+    if (st == KTM_OK) {
+        img = (float *)malloc(sizeof(float) * nimg);
+        if (!img) st = KTM_ERR_NOMEM;
+    }
+    if (st == KTM_OK) st = ktm_copy_to_host(img, b.image, sizeof(float) * nimg);
+    free_bufs(&b);
+    if (st != KTM_OK) { free(img); return st; }
+    /* image<job>.su: per (xc, yc, ih) a 240-byte header and ns big-endian
+     * IEEE floats, in the order of Kirchhoff.c:1440-1598 */
+    char name[4200];
+    snprintf(name, sizeof name, "%s/image%d.su", G.image_dir, job_id);
+    FILE *f = fopen(name, "wb");
+    if (!f) { free(img); return KTM_ERR_IO; }
+    const int xs = job_id / G.cfg.jobs_y, ys = job_id % G.cfg.jobs_y;
+    uint8_t hdr[240];
+    uint8_t *data = (uint8_t *)malloc((size_t)s.ns * 4);
+    if (!data) st = KTM_ERR_NOMEM;
+    for (int xc = 0; xc < s.nx && st == KTM_OK; ++xc)
+        for (int yc = 0; yc < s.ny && st == KTM_OK; ++yc)
+            for (int ih = 0; ih < s.nh && st == KTM_OK; ++ih) {
+                const size_t X = (size_t)xs * s.nx + xc, Y = (size_t)ys * s.ny + yc;
+                const float *tr = img + ((X * s.nyp + Y) * s.nh + ih) * s.ns;
+                ktm_segy_image_trace_header(&G.cfg, s.ns, s.dt, job_id, xc, yc, ih, hdr);
+                for (int t = 0; t < s.ns; ++t) {
+                    uint32_t u;
+                    memcpy(&u, tr + t, 4);
+                    ktm_put_be32(data + 4 * t, u);
+                }
+                if (fwrite(hdr, 1, 240, f) != 240 || fwrite(data, 4, (size_t)s.ns, f) != (size_t)s.ns)
+                    st = KTM_ERR_IO;
+            }
+    free(data);
+    free(img);
+    if (fclose(f) != 0 && st == KTM_OK) st = KTM_ERR_IO;
+    return st;
+}
+
+void kirchhoff(int job_id) { G.status = kirchhoff_job(job_id); }

@@ -1,0 +1,128 @@
+"""GPU parity of the FP32 reference path: HIP kernels vs the CPU oracle.
+
+Bar: bit-exact fold (integer counts), km within relative L2 1e-5 (the
+tolerance the migration path states) -- and, because the traveltime index is
+reproduced exactly and the image accumulates in trace order, the tests also
+report how many km samples are bit-identical (expected: all).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from random_amd import PREP_REFERENCE, TRACE_F32, surveys
+from random_amd.ktm import Plan
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def line0():
+    """Configuration 0: 40 shots x 120 receivers, ns=1001, 800 columns, 4 job slabs."""
+    sv = surveys.line2d(jobs_x=4)
+    tr = surveys.traces_numpy(sv)
+    return sv, tr
+
+
+@pytest.fixture(scope="module")
+def line0_oracle(line0):
+    sv, tr = line0
+    km, fold, n = oracle.migrate(sv.cfg, sv.ns, sv.dt, tr, sv.coords, sv.vel)
+    return km, fold, n
+
+
+def test_prep_reference_bitexact(cuda, line0):
+    sv, tr = line0
+    say("case: prep REFERENCE, line2d, 300 traces")
+    plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
+    sel = np.arange(0, sv.ntr, 16)[:300]
+    d_tr = torch.from_numpy(np.ascontiguousarray(tr[sel])).to(cuda)
+    d_co = torch.from_numpy(np.ascontiguousarray(sv.coords[sel])).to(cuda)
+    prepared, geom = plan.prep(d_tr, d_co, PREP_REFERENCE, TRACE_F32)
+    prepared = prepared.cpu().numpy()
+    geom = geom.cpu().numpy()
+    tm, _ = oracle.alias_filter(sv.cfg, sv.ns, sv.dt)
+    geo_o, off_o = oracle.trace_geometry(sv.cfg, sv.coords[sel])
+    assert np.array_equal(geom[:, :6].view(np.uint32), geo_o.view(np.uint32))
+    assert np.array_equal(geom[:, 6].view(np.int32), off_o)
+    for k, i in enumerate(sel):
+        F = oracle.filtered_trace(oracle.trace_prep(tr[i], sv.dt, tm), tm, sv.ns)
+        assert np.array_equal(prepared[k, :sv.ns].view(np.uint32), F.view(np.uint32)), k
+        assert not prepared[k, sv.ns:].any()
+
+
+def test_migrate_line0_bitexact(cuda, line0, line0_oracle):
+    sv, tr = line0
+    km_o, fold_o, n_o = line0_oracle
+    say("case: migrate line2d configuration 0, 4800 traces, ns=1001, 4 jobs")
+    plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
+    km, fold, n = plan.migrate(torch.from_numpy(tr).to(cuda), torch.from_numpy(sv.coords).to(cuda))
+    km = km.cpu().numpy()
+    fold = fold.cpu().numpy()
+    same = float(np.mean(km.view(np.uint32) == km_o.view(np.uint32)))
+    r = rel_l2(km, km_o)
+    say(f"  contributions {n} (oracle {n_o}); km relL2 {r:.3e}; bit-identical fraction {same:.8f}")
+    assert n == n_o
+    assert np.array_equal(fold, fold_o)
+    assert r <= 1e-5
+    assert same == 1.0
+
+
+def test_blocks_and_jobs_equal_single_pass(cuda, line0, line0_oracle):
+    sv, tr = line0
+    km_o, fold_o, _ = line0_oracle
+    say("case: line2d in blocks of 700 traces, and per-job regions")
+    plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
+    d_tr = torch.from_numpy(tr).to(cuda)
+    d_co = torch.from_numpy(sv.coords).to(cuda)
+    km, fold, _ = plan.migrate(d_tr, d_co, block=700)
+    assert np.array_equal(km.cpu().numpy().view(np.uint32), km_o.view(np.uint32))
+    km2, fold2, _ = plan.new_image()
+    prepared, geom = plan.prep(d_tr, d_co)
+    for job in range(sv.cfg.jobs_x):
+        plan.migrate_block(prepared, geom, km2, fold2, job=job)
+    assert np.array_equal(km2.cpu().numpy().view(np.uint32), km_o.view(np.uint32))
+    assert np.array_equal(fold2.cpu().numpy(), fold_o)
+
+
+def test_migrate_area3d_small_bitexact(cuda):
+    say("case: area3d small, 4000 traces, 40x40 columns, ns=1500, tm=8, 2x2 jobs")
+    sv = surveys.area3d(ntr=4000, ncols=40, extent=1000.0, max_offset=4000.0, nscat=12,
+                        jobs_x=2, jobs_y=2)
+    tr = surveys.traces_numpy(sv)
+    km_o, fold_o, n_o = oracle.migrate(sv.cfg, sv.ns, sv.dt, tr, sv.coords, sv.vel)
+    plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
+    assert plan.setup["trange_max"] == 8
+    km, fold, n = plan.migrate(torch.from_numpy(tr).to(cuda), torch.from_numpy(sv.coords).to(cuda),
+                               block=1500)
+    km = km.cpu().numpy()
+    same = float(np.mean(km.view(np.uint32) == km_o.view(np.uint32)))
+    r = rel_l2(km, km_o)
+    say(f"  contributions {n} (oracle {n_o}); km relL2 {r:.3e}; bit-identical {same:.8f}")
+    assert n == n_o
+    assert np.array_equal(fold.cpu().numpy(), fold_o)
+    assert r <= 1e-5
+    assert same == 1.0
+
+
+def test_poststack_matches_oracle(cuda, line0, line0_oracle):
+    sv, _ = line0
+    km_o, fold_o, _ = line0_oracle
+    say("case: post-stack filter on the configuration-0 image")
+    plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
+    out = plan.poststack(torch.from_numpy(km_o).to(cuda), torch.from_numpy(fold_o).to(cuda))
+    tm, aa = oracle.alias_filter(sv.cfg, sv.ns, sv.dt)
+    ref = oracle.poststack(km_o[::7], fold_o[::7], sv.dt, aa)
+    r = rel_l2(out.cpu().numpy()[::7], ref)
+    say(f"  image relL2 vs oracle (FFT in double, float spectrum) {r:.3e}")
+    assert r <= 1e-6
