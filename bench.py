@@ -50,7 +50,21 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy 3x (HBM counter calibration) and exit")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """A plain device copy of known size, for scaling FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    n = 1 << 28  # 2^28 floats = 1 GiB
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 3}), flush=True)
 
 
 def cpu_baseline(sv, traces_np, target_s, nthreads):
@@ -81,6 +95,9 @@ def cpu_baseline(sv, traces_np, target_s, nthreads):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU decomposition is "
                                    "described in DESIGN.md (image slabs need no collective)",

@@ -241,6 +241,17 @@ int ktm_synth_traces(const ktm_plan *plan, const ktm_trace_geom *geom, int64_t n
 /* traces[i] += sigma * N(0,1), the normal deviate a function of (seed, i). */
 int ktm_synth_noise(float *traces, int64_t n, float sigma, uint64_t seed, void *stream);
 
+/* ---- self-tests of the numerical assumptions of the kernels (device) ---- */
+
+/* out_hw[i] = the hardware square root v_sqrt_f32(x[i]) used by the fast
+ * traveltime path (its error bound is an input of that path's proof). */
+int ktm_selftest_sqrt(const float *x, int64_t n, float *out_hw, void *stream);
+/* in4[n][4] = (dIS2, dIR2, t2, v2): q_fast (fast path), q_ref = RN(dsr/dt)
+ * with the arithmetic of Kirchhoff.c:1381-1384, and the sample index the
+ * migration kernel selects (t_kernel) vs the reference's (t_ref). */
+int ktm_selftest_traveltime(const float *in4, int64_t n, float dt, int ns, float *q_fast,
+                            float *q_ref, int32_t *t_kernel, int32_t *t_ref, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

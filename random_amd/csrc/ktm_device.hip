@@ -259,12 +259,46 @@ __device__ __forceinline__ int exact_T(float dIS2, float dIR2, float t2, float v
     return qq < (float)ns ? (int)qq : ns;
 }
 
-/* Relative bound on |fast q - exact q| (see k_migrate): 11 units of 2^-24
- * from the analysis, rounded up to 1e-6. */
-#define KTM_FAST_EPS 1.0e-6f
+/* Fast path of the traveltime index.  With r = RN(1/v2) and rdt = RN(1/dt)
+ * precomputed,
+ *   q = RN( RN( sqrt_hw(fma(dIS2, r, t2)) + sqrt_hw(fma(dIR2, r, t2)) ) * rdt ).
+ * Against the reference value Q = RN(dsr/dt) of exact_T (u = 2^-24):
+ *   fma vs RN(t2 + RN(dIS2/v2))            relative 4u on the radicand
+ *   -> 2u after the square root, plus the hardware sqrt (<= 1 ulp = 2u)
+ *   sum rounding vs the double sum rounded to float         2u
+ *   RN(1/dt) and the product rounding vs RN(dsr/dt)         3u
+ * so |q - Q| <= 9u Q (first order).  KTM_FAST_EPS = 10u = 5.96e-7 rounded up;
+ * the 1 ulp figure of v_sqrt_f32 is checked exhaustively on [1, 4) by the GPU
+ * tests, the whole bound on random and near-integer inputs. */
+#define KTM_FAST_EPS 6.0e-7f
 
+__device__ __forceinline__ float fast_q(float dIS2, float dIR2, float r, float t2, float rdt)
+{
+    const float a1 = __builtin_fmaf(dIS2, r, t2);
+    const float a2 = __builtin_fmaf(dIR2, r, t2);
+    const float d = __builtin_amdgcn_sqrtf(a1) + __builtin_amdgcn_sqrtf(a2);
+    return d * rdt;
+}
+
+/* An integer within KTM_FAST_EPS*(q+1) of q: floor(q) may differ from the
+ * reference's floor(Q).  Beyond ns+1 both are >= ns (sample discarded). */
+__device__ __forceinline__ bool fast_ambiguous(float q, float nsf1)
+{
+    const float dev = fabsf(q - __builtin_rintf(q));
+    return q < nsf1 && dev <= __builtin_fmaf(q, KTM_FAST_EPS, KTM_FAST_EPS);
+}
+
+/* floor(q) clamped to [0, ns] (q >= 0) */
+__device__ __forceinline__ int fast_T(float q, float nsf)
+{
+    return (int)__builtin_amdgcn_fmed3f(q, 0.0f, nsf);
+}
+
+/* The plain-FP32 variants fit 128 VGPRs, i.e. 4 waves per SIMD: two
+ * 8-wave workgroups per CU, so one stages traces while the other computes. */
 template <int K, int W, int B, bool KAHAN, int SRC>
 __global__ void __launch_bounds__(8 * KTM_WAVE)
+__attribute__((amdgpu_waves_per_eu(KAHAN ? 1 : 4, KAHAN ? 4 : 4)))
 k_migrate(const MigArgs a)
 {
     constexpr int NCOL = 8 / W;             /* image columns per workgroup */
@@ -276,7 +310,8 @@ k_migrate(const MigArgs a)
     float *sG = sF + B * ldl;                               /* [B][8]   */
     int *sSel = reinterpret_cast<int *>(sG + B * 8);        /* [NT]     */
     int *sCnt = sSel + NT;                                  /* [NW]     */
-    float *sW = reinterpret_cast<float *>(sCnt + NW);       /* [ns+2] (BF16/F16 only) */
+    float *sV2 = reinterpret_cast<float *>(sCnt + NW);      /* [NCOL][W*K*64] v^2 per sample */
+    float *sW = sV2 + NCOL * W * K * KTM_WAVE;              /* [ns+2] (BF16/F16 only) */
 
     const int tid = threadIdx.x, lane = tid & (KTM_WAVE - 1), wv = tid / KTM_WAVE;
     /* XCD-aware tile order: consecutive workgroup ids go to different XCDs,
@@ -314,22 +349,25 @@ k_migrate(const MigArgs a)
     /* per-sample constants of this wave's column segment */
     const int64_t pix = (((int64_t)Xc * a.nyp + Yc) * a.nh + h) * ns;
     const int64_t vbase = ((int64_t)a.velx[Xc] * a.nvel_xl + a.vely[Yc]) * a.nsm;
-    float r[K], t2v[K], v2v[K], acc[K], cmp[K];
+    /* v^2 of this column for the exact path lives in LDS; the fast path keeps
+     * 1/v^2 and t2 in registers */
+    float *sV2c = sV2 + cw * (W * K * KTM_WAVE);
+    float r[K], t2v[K], acc[K], cmp[K];
     int cnt[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const int t = seg * K * KTM_WAVE + k * KTM_WAVE + lane;
+        const int tl = seg * K * KTM_WAVE + k * KTM_WAVE + lane;
         cnt[k] = 0;
         cmp[k] = 0.0f;
-        if (t < ns) {
-            const float v2 = a.vel2[vbase + a.t1[t]];
-            v2v[k] = v2;
+        if (tl < ns) {
+            const float v2 = a.vel2[vbase + a.t1[tl]];
+            sV2c[tl] = v2;
             r[k] = 1.0f / v2;
-            t2v[k] = a.t2[t];
-            acc[k] = col_ok ? a.km[pix + t] : 0.0f;
-            if (KAHAN && col_ok) cmp[k] = a.comp[pix + t];
+            t2v[k] = a.t2[tl];
+            acc[k] = col_ok ? a.km[pix + tl] : 0.0f;
+            if (KAHAN && col_ok) cmp[k] = a.comp[pix + tl];
         } else {
-            v2v[k] = 1.0f;
+            sV2c[tl] = 1.0f;
             r[k] = 0.0f;
             t2v[k] = 3.0e38f;   /* q becomes huge: T = ns, nothing added */
             acc[k] = 0.0f;
@@ -436,14 +474,11 @@ k_migrate(const MigArgs a)
                      * always the reference's. */
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
-                        const float a1 = __builtin_fmaf(dIS2, r[k], t2v[k]);
-                        const float a2 = __builtin_fmaf(dIR2, r[k], t2v[k]);
-                        const float d = __builtin_amdgcn_sqrtf(a1) + __builtin_amdgcn_sqrtf(a2);
-                        const float q = d * a.rdt;
-                        int T = (int)fminf(q, nsf);
-                        const float dev = fabsf(q - __builtin_rintf(q));
-                        if (q < nsf1 && dev <= __builtin_fmaf(q, KTM_FAST_EPS, KTM_FAST_EPS))
-                            T = exact_T(dIS2, dIR2, t2v[k], v2v[k], a.dt, ns);
+                        const float q = fast_q(dIS2, dIR2, r[k], t2v[k], a.rdt);
+                        int T = fast_T(q, nsf);
+                        if (fast_ambiguous(q, nsf1))
+                            T = exact_T(dIS2, dIR2, t2v[k],
+                                        sV2c[seg * K * KTM_WAVE + k * KTM_WAVE + lane], a.dt, ns);
                         const float f = Fr[T];
                         if (KAHAN) {
                             const float y = f - cmp[k];
@@ -643,6 +678,61 @@ extern "C" int ktm_synth_noise(float *traces, int64_t n, float sigma, uint64_t s
     return KTM_OK;
 }
 
+
+/* ------------------------------------------------------------------------ */
+/* self-tests of the numerical assumptions of the fast traveltime path       */
+/* ------------------------------------------------------------------------ */
+/* out_hw[i] = v_sqrt_f32(x[i]) */
+__global__ void k_selftest_sqrt(const float *__restrict__ x, int64_t n, float *__restrict__ out_hw)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out_hw[i] = __builtin_amdgcn_sqrtf(x[i]);
+}
+
+/* For in[i] = (dIS2, dIR2, t2, v2): q_fast, the reference Q = RN(dsr/dt)
+ * (float, before the floor), and the T each path selects in the kernel. */
+__global__ void k_selftest_traveltime(const float4 *__restrict__ in, int64_t n, float dt, int ns,
+                                      float *__restrict__ q_fast, float *__restrict__ q_ref,
+                                      int *__restrict__ t_kernel, int *__restrict__ t_ref)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 v = in[i];
+    const float r = 1.0f / v.w, rdt = 1.0f / dt;
+    const float q = fast_q(v.x, v.y, r, v.z, rdt);
+    const float nsf = (float)ns, nsf1 = (float)(ns + 1);
+    int T = fast_T(q, nsf);
+    if (fast_ambiguous(q, nsf1)) T = exact_T(v.x, v.y, v.z, v.w, dt, ns);
+    const float s1 = v.z + v.x / v.w, s2 = v.z + v.y / v.w;
+    const float dsr = (float)(__builtin_sqrt((double)s1) + __builtin_sqrt((double)s2));
+    q_fast[i] = q;
+    q_ref[i] = dsr / dt;
+    t_kernel[i] = T;
+    t_ref[i] = exact_T(v.x, v.y, v.z, v.w, dt, ns);
+}
+
+extern "C" int ktm_selftest_sqrt(const float *x, int64_t n, float *out_hw, void *stream)
+{
+    if (n < 0 || (n && (!x || !out_hw))) return KTM_ERR_ARG;
+    if (!n) return KTM_OK;
+    hipLaunchKernelGGL(k_selftest_sqrt, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, x, n, out_hw);
+    HIP_TRY(hipGetLastError());
+    return KTM_OK;
+}
+
+extern "C" int ktm_selftest_traveltime(const float *in4, int64_t n, float dt, int ns, float *q_fast,
+                                       float *q_ref, int32_t *t_kernel, int32_t *t_ref, void *stream)
+{
+    if (n < 0 || (n && (!in4 || !q_fast || !q_ref || !t_kernel || !t_ref))) return KTM_ERR_ARG;
+    if (!n) return KTM_OK;
+    hipLaunchKernelGGL(k_selftest_traveltime, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, (const float4 *)in4, n, dt, ns, q_fast, q_ref,
+                       t_kernel, t_ref);
+    HIP_TRY(hipGetLastError());
+    return KTM_OK;
+}
+
 /* ------------------------------------------------------------------------ */
 /* host glue                                                                 */
 /* ------------------------------------------------------------------------ */
@@ -820,13 +910,13 @@ static mig_fn pick_src(int dtype, int accum)
 
 /* W waves share one image column (each owns K*64 consecutive samples);
  * a workgroup is 8 waves, so it covers 8/W columns. */
-static mig_fn pick_migrate(int ns, int dtype, int accum, int *W)
+static mig_fn pick_migrate(int ns, int dtype, int accum, int *W, int *K)
 {
-    if (ns <= 512) { *W = 1; return pick_src<8, 1>(dtype, accum); }
-    if (ns <= 1024) { *W = 2; return pick_src<8, 2>(dtype, accum); }
-    if (ns <= 1536) { *W = 2; return pick_src<12, 2>(dtype, accum); }
-    if (ns <= 2048) { *W = 4; return pick_src<8, 4>(dtype, accum); }
-    *W = 4;
+    if (ns <= 512) { *W = 1; *K = 8; return pick_src<8, 1>(dtype, accum); }
+    if (ns <= 1024) { *W = 2; *K = 8; return pick_src<8, 2>(dtype, accum); }
+    if (ns <= 1536) { *W = 2; *K = 12; return pick_src<12, 2>(dtype, accum); }
+    if (ns <= 2048) { *W = 4; *K = 8; return pick_src<8, 4>(dtype, accum); }
+    *W = 4; *K = 12;
     return pick_src<12, 4>(dtype, accum);
 }
 
@@ -864,8 +954,8 @@ extern "C" int ktm_migrate_block(const ktm_plan *p, const void *prepared, int dt
         a.rx0 = (job / p->cfg.jobs_y) * s->nx; a.rx1 = a.rx0 + s->nx;
         a.ry0 = (job % p->cfg.jobs_y) * s->ny; a.ry1 = a.ry0 + s->ny;
     }
-    int W = 1;
-    mig_fn fn = pick_migrate(s->ns, dtype, accum, &W);
+    int W = 1, K = 8;
+    mig_fn fn = pick_migrate(s->ns, dtype, accum, &W, &K);
     /* tile of 8/W columns: a row on a 2-D line, as square as possible in 3-D */
     const int ncol = 8 / W;
     a.tx_cols = s->nyp == 1 ? ncol : (ncol == 8 ? 4 : ncol == 4 ? 2 : ncol);
@@ -875,6 +965,7 @@ extern "C" int ktm_migrate_block(const ktm_plan *p, const void *prepared, int dt
     a.ntiles = a.tiles_x * tiles_y;
     const int nthreads = 8 * KTM_WAVE;
     const size_t lds = sizeof(float) * (size_t)(8 * p->row + 8 * 8) + sizeof(int) * (size_t)(nthreads + 8) +
+                       sizeof(float) * (size_t)(8 * K * KTM_WAVE) +
                        (dtype != KTM_TRACE_F32 ? sizeof(float) * (size_t)(s->ns + 2) : 0);
     if (lds > 160 * 1024) return KTM_ERR_UNSUPPORTED;
     const int per = (a.ntiles + 7) / 8;
