@@ -33,6 +33,10 @@ HBM_PEAK_GBS = 8000.0
 BYTES_FP32 = 12.0   # three FP32 trace taps per contribution
 BYTES_HALF = 6.0    # three 16-bit taps
 METRIC = "G trace×image-pt contributions/s, %HBM roofline; BF16-trace vs FP32 at err"
+# HBM bytes per launch of k_migrate on this workload from rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE (profiles/k_migrate_area3d_1e6.md), fetch scaled x2 and
+# write x1 by the same counters on a 1 GiB copy; None until measured
+TRAFFIC = None
 
 
 def log(*a):
@@ -50,6 +54,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3")
+    ap.add_argument("--leg-steps", type=int, default=3, help="timed blocks per precision leg")
+    ap.add_argument("--err-traces", type=int, default=20000,
+                    help="traces of the FP64-oracle error check on the 50x50x1500 sub-volume")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy 3x (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -70,27 +77,78 @@ def copy_calibration():
 def cpu_baseline(sv, traces_np, target_s, nthreads):
     """Oracle FP32 (the reference algorithm, strict IEEE, -O3) on the first S
     traces over the whole image, jobs = one slab per thread as the reference's
-    MPI decomposition.  S is calibrated so the run takes about target_s."""
+    MPI decomposition.  S is calibrated so the run takes about target_s.  The
+    same restatement built with the reference's own flags (-O0 -ffast-math,
+    Kirchhoff.c:3) is timed on a quarter of the sample beside it."""
     import oracle
     cfg = sv.cfg.replace(jobs_x=nthreads, jobs_y=1)
-    n_cal = 16
+    n_cal = 48
     t0 = time.perf_counter()
     _, _, nc = oracle.migrate(cfg, sv.ns, sv.dt, traces_np[:n_cal], sv.coords[:n_cal], sv.vel,
                               nthreads=nthreads)
     el = time.perf_counter() - t0
-    rate = nc / max(el, 1e-9)
-    per_trace = nc / n_cal
-    S = int(max(n_cal, min(traces_np.shape[0], target_s * rate / max(per_trace, 1))))
+    S = int(max(n_cal, min(traces_np.shape[0], n_cal * target_s / max(el, 1e-6))))
     log(f"cpu baseline: calibration {nc:.3e} contrib in {el:.2f}s; sample {S} traces")
     t0 = time.perf_counter()
     _, _, nc = oracle.migrate(cfg, sv.ns, sv.dt, traces_np[:S], sv.coords[:S], sv.vel,
                               nthreads=nthreads)
     el = time.perf_counter() - t0
-    return {"value": nc / el / 1e9, "unit": "G contributions/s", "cores": nthreads,
+    S4 = max(n_cal, S // 4)
+    t1 = time.perf_counter()
+    _, _, nc4 = oracle.migrate(cfg, sv.ns, sv.dt, traces_np[:S4], sv.coords[:S4], sv.vel,
+                               nthreads=nthreads, refflags=True)
+    el4 = time.perf_counter() - t1
+    log(f"cpu baseline: {nc:.3e} contrib in {el:.1f}s; reference flags {nc4:.3e} in {el4:.1f}s")
+    return {"value": round(nc / el / 1e9, 4), "unit": "G contributions/s", "cores": nthreads,
             "kind": "port",
             "sample": f"first {S} of {sv.ntr} traces over the whole {sv.cfg.end_iline}x"
-                      f"{sv.cfg.end_xline}x{sv.ns} image, {nc:.3e} contributions in {el:.1f} s, "
-                      f"{nthreads} job slabs (oracle/ktm_oracle.c, gcc -O3 -march=x86-64-v3, strict IEEE)"}
+                      f"{sv.cfg.end_xline}x{sv.ns} image, {nc:.3e} contributions in {el:.1f} s on "
+                      f"{nthreads} threads, one job slab each (oracle/ktm_oracle.c, gcc -O3 "
+                      f"-march=x86-64-v3, strict IEEE)",
+            "reference_flags_value": round(nc4 / el4 / 1e9, 4),
+            "reference_flags_sample": f"first {S4} traces, same restatement built with -O0 "
+                                      f"-ffast-math (Kirchhoff.c:3), {el4:.1f} s"}
+
+
+def precision_errors(sv, plan, traces, coords, ntr, dev):
+    """Errors of every path against the FP64 oracle (FP32 traveltime index,
+    FP64 data path) on the 50 x 50 x 1500 sub-volume at columns 175..224 of
+    both axes, migrating the first ntr traces of the survey."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from random_amd import (ACCUM_KAHAN, ACCUM_PLAIN, PREP_CLOSED, PREP_REFERENCE, TRACE_BF16,
+                            TRACE_F16, TRACE_F32)
+    from random_amd.ktm import Plan
+    cfg = sv.cfg.replace(start_iline=175, end_iline=225, start_xline=175, end_xline=225,
+                         jobs_x=10, jobs_y=2)
+    sub = Plan(cfg, sv.ns, sv.dt, sv.vel)
+    tr_np = traces[:ntr].cpu().numpy()
+    co_np = sv.coords[:ntr]
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    km64, _, n64 = oracle.migrate(cfg, sv.ns, sv.dt, tr_np, co_np, sv.vel, mode=oracle.FP64_DATA,
+                                  nthreads=nthreads)
+    log(f"fp64 oracle on the sub-volume: {n64:.3e} contributions in "
+        f"{time.perf_counter() - t0:.1f}s")
+    out = {"error_check": f"50x50x{sv.ns} sub-volume (columns 175-224 x 175-224), first {ntr} "
+                          f"traces, {n64:.3e} contributions; reference = CPU oracle with FP64 data "
+                          f"path and the reference FP32 traveltime index"}
+    nrm = float(np.linalg.norm(km64))
+    mx = float(np.abs(km64).max())
+    for label, mode, dtype, accum in (
+            ("fp32", PREP_REFERENCE, TRACE_F32, ACCUM_PLAIN),
+            ("fp32_closed_kahan", PREP_CLOSED, TRACE_F32, ACCUM_KAHAN),
+            ("fp16", PREP_CLOSED, TRACE_F16, ACCUM_KAHAN),
+            ("bf16", PREP_CLOSED, TRACE_BF16, ACCUM_KAHAN)):
+        km, _, n = sub.migrate(traces[:ntr], coords[:ntr], mode=mode, dtype=dtype, accum=accum)
+        d = km.cpu().numpy().astype(np.float64) - km64
+        e = {"rel_l2_vs_fp64": float(np.linalg.norm(d) / nrm),
+             "max_abs_vs_fp64_rel_to_max": float(np.abs(d).max() / mx)}
+        out.setdefault(label, {}).update(e)
+        log(f"{label}: {e}")
+    return out
 
 
 def main():
@@ -106,7 +164,8 @@ def main():
     import numpy as np
     import torch
 
-    from random_amd import PREP_REFERENCE, TRACE_F32, surveys, synth
+    from random_amd import (ACCUM_KAHAN, ACCUM_PLAIN, PREP_CLOSED, PREP_REFERENCE, TRACE_BF16,
+                            TRACE_F16, TRACE_F32, surveys, synth)
     from random_amd.ktm import Plan
 
     dev = torch.device("cuda:0")
@@ -121,42 +180,50 @@ def main():
         f"{plan.image_shape}, tm={plan.setup['trange_max']}")
 
     nblocks = (sv.ntr + args.block - 1) // args.block
-    steps = args.steps
-    warm = args.warmup
-
-    def run_block(b):
-        i0 = (b % nblocks) * args.block
-        sl = slice(i0, min(i0 + args.block, sv.ntr))
-        prepared, geom = plan.prep(traces[sl], coords[sl], PREP_REFERENCE, TRACE_F32)
-        e_mig0.record()
-        plan.migrate_block(prepared, geom, km, fold, counter=counter)
-        e_mig1.record()
-
     e_mig0 = torch.cuda.Event(enable_timing=True)
     e_mig1 = torch.cuda.Event(enable_timing=True)
-    for b in range(warm):
-        run_block(b)
-        torch.cuda.synchronize()
-        log(f"warmup block {b}: migrate {e_mig0.elapsed_time(e_mig1):.1f} ms")
-    plan.reset(km, fold)
-    counter.zero_()
-    torch.cuda.synchronize()
 
-    mig_ms = []
-    e_start = torch.cuda.Event(enable_timing=True)
-    e_end = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e_start.record()
-    for b in range(steps):
-        run_block(b)
-        e_mig1.synchronize()
-        mig_ms.append(e_mig0.elapsed_time(e_mig1))
-        log(f"step {b}: migrate kernel {mig_ms[-1]:.1f} ms")
-    e_end.record()
-    torch.cuda.synchronize()
-    total_ms = e_start.elapsed_time(e_end)
-    n_contrib = int(counter.item())
-    gcs = n_contrib / (total_ms * 1e-3) / 1e9
+    def run_mode(steps, warm, mode, dtype, accum, label):
+        """warm + steps trace blocks into a fresh image; returns (G contrib/s of
+        the timed steps incl. prep, per-launch migrate ms list, contributions)."""
+        km, fold, comp = plan.new_image(kahan=accum == ACCUM_KAHAN)
+        counter = torch.zeros(1, dtype=torch.int64, device=dev)
+
+        def run_block(b):
+            i0 = (b % nblocks) * args.block
+            sl = slice(i0, min(i0 + args.block, sv.ntr))
+            prepared, geom = plan.prep(traces[sl], coords[sl], mode, dtype)
+            e_mig0.record()
+            plan.migrate_block(prepared, geom, km, fold, comp, dtype=dtype, mode=mode,
+                               accum=accum, counter=counter)
+            e_mig1.record()
+
+        for b in range(warm):
+            run_block(b)
+            torch.cuda.synchronize()
+            log(f"{label} warmup block {b}: migrate {e_mig0.elapsed_time(e_mig1):.1f} ms")
+        plan.reset(km, fold, comp)
+        counter.zero_()
+        mig_ms = []
+        e_start = torch.cuda.Event(enable_timing=True)
+        e_end = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e_start.record()
+        for b in range(steps):
+            run_block(b)
+            e_mig1.synchronize()
+            mig_ms.append(e_mig0.elapsed_time(e_mig1))
+            log(f"{label} step {b}: migrate kernel {mig_ms[-1]:.1f} ms")
+        e_end.record()
+        torch.cuda.synchronize()
+        total_ms = e_start.elapsed_time(e_end)
+        n = int(counter.item())
+        del km, fold, comp
+        return n / (total_ms * 1e-3) / 1e9, mig_ms, n, total_ms
+
+    steps, warm = args.steps, args.warmup
+    gcs, mig_ms, n_contrib, total_ms = run_mode(steps, warm, PREP_REFERENCE, TRACE_F32,
+                                                ACCUM_PLAIN, "fp32")
     per_launch = n_contrib / steps
     achieved = per_launch * BYTES_FP32 / (np.mean(mig_ms) * 1e-3) / 1e9
 
@@ -171,10 +238,29 @@ def main():
                    "traces": sv.ntr, "ns": sv.ns, "image": list(plan.image_shape[:2]) + [sv.ns],
                    "block_traces": args.block, "contributions": n_contrib},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC,
                      "kernel": "k_migrate", "bytes_per_contribution": BYTES_FP32,
                      "avg_launch_ms": round(float(np.mean(mig_ms)), 2)},
     }
+    if not args.profile and args.leg_steps > 0:
+        legs = {}
+        for label, mode, dtype, accum, bpc in (
+                ("bf16", PREP_CLOSED, TRACE_BF16, ACCUM_KAHAN, BYTES_HALF),
+                ("fp16", PREP_CLOSED, TRACE_F16, ACCUM_KAHAN, BYTES_HALF),
+                ("fp32_closed_kahan", PREP_CLOSED, TRACE_F32, ACCUM_KAHAN, BYTES_FP32)):
+            g, ms, n, _ = run_mode(args.leg_steps, 1, mode, dtype, accum, label)
+            legs[label] = {"value": round(g, 3), "speedup_vs_fp32": round(g / gcs, 3),
+                           "avg_launch_ms": round(float(np.mean(ms)), 2),
+                           "roofline_frac": round(n / len(ms) * bpc / (np.mean(ms) * 1e-3) / 1e9
+                                                  / HBM_PEAK_GBS, 4),
+                           "bytes_per_contribution": bpc}
+        if not args.no_cpu and args.err_traces > 0:
+            for k, v in precision_errors(sv, plan, traces, coords, args.err_traces, dev).items():
+                if isinstance(v, dict):
+                    legs.setdefault(k, {}).update(v)
+                else:
+                    legs[k] = v
+        result["mixed_precision"] = legs
     if not args.no_cpu and not args.profile:
         nthreads = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
         nthreads = max(1, min(nthreads, 64))

@@ -1,0 +1,97 @@
+"""The reference-signature path: kirchhoff(job_id) reads a SEG-Y file, migrates
+its job slab on the GPU and writes image_dir/image<job>.su (Kirchhoff.c:
+1104-1616), plus the host one-call driver ktm_migrate."""
+import ctypes as C
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import oracle
+from random_amd import _lib, surveys
+from random_amd.ktm import migrate_host
+
+pytestmark = pytest.mark.gpu
+
+
+def vp(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def write_input(path, sv, tr):
+    """IBM-format SEG-Y with sx, sy, gx, gy in words 18-21 (bytes 73-88)."""
+    hdrs = np.zeros((sv.ntr, 240), np.uint8)
+    for i in range(sv.ntr):
+        hdrs[i, 72:88] = np.frombuffer(struct.pack(">4i", *map(int, sv.coords[i])), np.uint8)
+        hdrs[i, 114:118] = np.frombuffer(struct.pack(">HH", sv.ns, 4000), np.uint8)
+    fh = np.zeros(3600, np.uint8)
+    _lib.check(_lib.lib().ktm_segy_write(path.encode(), vp(fh), vp(hdrs), vp(tr), sv.ntr, sv.ns, 0),
+               "ktm_segy_write")
+
+
+def test_kirchhoff_job_files(cuda, tmp_path):
+    print("case: kirchhoff(job) on a 2-D line, 3 jobs, IBM input", flush=True)
+    sv = surveys.line2d(nshots=6, ncols=96, ns=501, nscat=4, jobs_x=3)
+    sv.cfg.ntblock = 250  # several read blocks, last one short
+    tr = surveys.traces_numpy(sv)
+    path = str(tmp_path / "in.sgy")
+    write_input(path, sv, tr)
+    # the reference decodes IBM: migrate exactly what it would have read
+    raw = np.fromfile(path, np.uint8)[3600:].reshape(sv.ntr, 240 + 4 * sv.ns)[:, 240:]
+    tr_ibm = oracle.ibm_to_float(raw.ravel()).reshape(sv.ntr, sv.ns)
+    L = _lib.lib()
+    c = sv.cfg.to_c()
+    img_dir = str(tmp_path / "image_dir")
+    _lib.check(L.ktm_globals_init(C.byref(c), path.encode(), None, vp(sv.vel), sv.ns,
+                                  np.float32(sv.dt), img_dir.encode()), "ktm_globals_init")
+    km_o, fold_o, _ = oracle.migrate(sv.cfg, sv.ns, sv.dt, tr_ibm, sv.coords, sv.vel)
+    tm, aa = oracle.alias_filter(sv.cfg, sv.ns, sv.dt)
+    img_o = oracle.poststack(km_o, fold_o, sv.dt, aa)
+    g = oracle.grid(sv.cfg)
+    try:
+        for job in range(g.no_job):
+            L.kirchhoff(job)
+            assert L.ktm_last_status() == 0
+            data = open(os.path.join(img_dir, f"image{job}.su"), "rb").read()
+            rec = 240 + 4 * sv.ns
+            assert len(data) == g.nx * g.ny * sv.cfg.nh * rec
+            for k in range(g.nx):
+                r = data[k * rec:(k + 1) * rec]
+                tracl = g.nx * g.ny * job + 1 + k
+                assert r[:240] == oracle.segy_trace_header(sv.cfg, sv.ns, sv.dt, job, k, 0, 0, tracl)
+                got = np.frombuffer(r[240:], ">f4").astype(np.float32)
+                ref = img_o[job * g.nx + k, 0, 0]
+                assert np.linalg.norm(got - ref) <= 1e-5 * max(np.linalg.norm(ref), 1e-30)
+        fh = open(os.path.join(img_dir, "binary_header.bin"), "rb").read()
+        assert fh == oracle.segy_file_header(sv.cfg, sv.ns, sv.dt)[3200:]
+        L.kirchhoff(g.no_job)  # a job that does not exist is an error, not an exit
+        assert L.ktm_last_status() == _lib.KTM_ERR_ARG
+    finally:
+        L.ktm_globals_free()
+
+
+def test_migrate_host_driver(cuda):
+    print("case: ktm_migrate host driver vs oracle", flush=True)
+    sv = surveys.line2d(nshots=4, ncols=120, ns=401, nscat=3, jobs_x=2)
+    sv.cfg.ntblock = 100
+    tr = surveys.traces_numpy(sv)
+    km, fold, img, st = migrate_host(sv.cfg, sv.ns, sv.dt, tr, sv.coords, sv.vel)
+    km_o, fold_o, n_o = oracle.migrate(sv.cfg, sv.ns, sv.dt, tr, sv.coords, sv.vel)
+    assert np.array_equal(km.view(np.uint32), km_o.view(np.uint32))
+    assert np.array_equal(fold, fold_o)
+    assert st["contributions"] == n_o
+
+
+def test_plan_rejects_bad_velocity(cuda):
+    from random_amd.ktm import Plan
+    sv = surveys.line2d(nshots=1, ncols=16, ns=101, nscat=1)
+    vel = sv.vel.copy()
+    vel[3, 0, 10] = 0.0
+    with pytest.raises(_lib.KtmError) as e:
+        Plan(sv.cfg, sv.ns, sv.dt, vel)
+    assert e.value.status == _lib.KTM_ERR_VELOCITY
+    cfg = sv.cfg.replace(ovel_ilines=5)  # columns 0..4 would index below the model
+    with pytest.raises(_lib.KtmError) as e:
+        Plan(cfg, sv.ns, sv.dt, sv.vel)
+    assert e.value.status == _lib.KTM_ERR_VELOCITY
