@@ -141,7 +141,8 @@ def precision_errors(sv, plan, traces, coords, ntr, dev):
             ("fp32", PREP_REFERENCE, TRACE_F32, ACCUM_PLAIN),
             ("fp32_closed_kahan", PREP_CLOSED, TRACE_F32, ACCUM_KAHAN),
             ("fp16", PREP_CLOSED, TRACE_F16, ACCUM_KAHAN),
-            ("bf16", PREP_CLOSED, TRACE_BF16, ACCUM_KAHAN)):
+            ("bf16", PREP_CLOSED, TRACE_BF16, ACCUM_KAHAN),
+            ("bf16_plain", PREP_CLOSED, TRACE_BF16, ACCUM_PLAIN)):
         km, _, n = sub.migrate(traces[:ntr], coords[:ntr], mode=mode, dtype=dtype, accum=accum)
         d = km.cpu().numpy().astype(np.float64) - km64
         e = {"rel_l2_vs_fp64": float(np.linalg.norm(d) / nrm),
@@ -247,6 +248,7 @@ def main():
         for label, mode, dtype, accum, bpc in (
                 ("bf16", PREP_CLOSED, TRACE_BF16, ACCUM_KAHAN, BYTES_HALF),
                 ("fp16", PREP_CLOSED, TRACE_F16, ACCUM_KAHAN, BYTES_HALF),
+                ("bf16_plain", PREP_CLOSED, TRACE_BF16, ACCUM_PLAIN, BYTES_HALF),
                 ("fp32_closed_kahan", PREP_CLOSED, TRACE_F32, ACCUM_KAHAN, BYTES_FP32)):
             g, ms, n, _ = run_mode(args.leg_steps, 1, mode, dtype, accum, label)
             legs[label] = {"value": round(g, 3), "speedup_vs_fp32": round(g / gcs, 3),

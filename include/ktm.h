@@ -130,9 +130,11 @@ int64_t ktm_plan_image_size(const ktm_plan *plan);
 /* Trace preparation (device): traces[ntr][ns] IEEE samples, coords[ntr][4]
  * header words sx,sy,gx,gy (bytes 73-88, host order, >= 0).  Writes geom[ntr]
  * and, for F32 storage, prepared[ntr][trace_row] floats holding F(T) for
- * T < ns (zero beyond); for BF16/F16 storage, prepared holds the raw trace
- * quantised with a per-trace max-abs scale (geom.scale) and the filter runs
- * inside the migration kernel.  Replaces Kirchhoff.c:1296-1328. */
+ * T < ns (zero beyond).  For BF16/F16 storage (CLOSED prep only), prepared
+ * [ntr][trace_row] holds the raw trace divided by its max-abs (geom.scale),
+ * rounded with first-order error feedback; ktm_migrate_block decodes and
+ * filters such a block into an FP32 workspace owned by the plan before
+ * migrating it.  Replaces Kirchhoff.c:1296-1328. */
 int ktm_prep(const ktm_plan *plan, const float *traces, const int32_t *coords,
              int64_t ntr, int prep_mode, int trace_dtype, void *prepared,
              ktm_trace_geom *geom, void *stream);
@@ -240,6 +242,12 @@ int ktm_synth_traces(const ktm_plan *plan, const ktm_trace_geom *geom, int64_t n
                      float *traces, void *stream);
 /* traces[i] += sigma * N(0,1), the normal deviate a function of (seed, i). */
 int ktm_synth_noise(float *traces, int64_t n, float sigma, uint64_t seed, void *stream);
+
+/* The float t such that, for every float x >= 0, x < t exactly when
+ * (double)(float)sqrt((double)x) < A -- the form of the aperture and slab
+ * gates of Kirchhoff.c:1334-1338,1352-1354 -- so the kernels gate on squared
+ * distances with the reference's decisions. */
+float ktm_gate_threshold(double A);
 
 /* ---- self-tests of the numerical assumptions of the kernels (device) ---- */
 

@@ -5,8 +5,9 @@
  * Kernels (every one replaces a loop of Kirchhoff.c's kirchhoff()):
  *   k_geometry      per-trace rotation / CMP / offset class   Kirchhoff.c:1296-1310
  *   k_prep_ref      FP32 double integral + trange=2 taps       Kirchhoff.c:1312-1328,1391-1392
- *   k_prep_closed   same operator as a 3-tap filter (FP32)     (exact rewrite of 1312-1392)
- *   k_quantize      BF16/FP16 storage with per-trace scale
+ *   k_prep_closed   same operator as a 3-tap filter (FP32), from FP32 or
+ *                   BF16/FP16-stored traces              (exact rewrite of 1312-1392)
+ *   k_quantize      BF16/FP16 storage with per-trace scale and error feedback
  *   k_migrate       the image loop, Kirchhoff.c:1332-1404: one workgroup owns a
  *                   tile of image columns (one wave per column, all ns samples
  *                   in registers), streams the traces of a block through LDS in
@@ -43,10 +44,12 @@ struct ktm_plan {
     float *d_colx, *d_coly;
     int32_t *d_velx, *d_vely;
     float *d_slab_cx, *d_slab_cy;
-    double *d_slab_thr;
+    float *d_slab_x2thr;
     float *d_vel2;          /* [nvel_il][nvel_xl][nsm], squared like Kirchhoff.c:617 */
     float *d_w;             /* [ns+2] CLOSED weights */
     double *d_hconv;        /* [ns]   post-stack kernel */
+    float *d_stage;         /* FP32 filtered traces of a BF16/FP16 block */
+    size_t stage_cap;       /* elements of d_stage */
 };
 
 #define HIP_TRY(expr)                                  \
@@ -150,6 +153,14 @@ __global__ void k_prep_ref(const float *__restrict__ traces, int64_t ntr, int ns
  *   T == 0 : 2(p[2] - p[0])        = 4x[0] + 4x[1] + 2x[2]
  * Shared by the F32 prep kernel and the BF16/FP16 staging in k_migrate so the
  * two storage modes differ only by quantisation. */
+__device__ __forceinline__ float bf16_to_f32_(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ float f16_to_f32_(uint16_t u)
+{
+    _Float16 h;
+    __builtin_memcpy(&h, &u, 2);
+    return (float)h;
+}
+
 __device__ __forceinline__ float closed_tap(int T, float x0, float x1, float x2, float x3)
 {
     /* for T >= 2, x0..x2 are x[T], x[T+1], x[T+2]; for T < 2 they are x[0..3] */
@@ -158,21 +169,34 @@ __device__ __forceinline__ float closed_tap(int T, float x0, float x1, float x2,
     return (4.0f * x0 + 4.0f * x1) + 2.0f * x2;
 }
 
-__global__ void k_prep_closed(const float *__restrict__ traces, int64_t ntr, int ns, int64_t row,
-                              const float *__restrict__ w, float *__restrict__ out)
+/* src: F32 raw traces [ntr][ns] (SRC = KTM_TRACE_F32) or BF16/FP16 traces
+ * stored by k_quantize [ntr][row] with the per-trace scale in geom; the
+ * stored value times the scale stands for the trace sample. */
+template <int SRC>
+__global__ void k_prep_closed(const void *__restrict__ src, const ktm_trace_geom *__restrict__ geom,
+                              int64_t ntr, int ns, int64_t row, const float *__restrict__ w,
+                              float *__restrict__ out)
 {
     const int64_t i = blockIdx.y;
     const int T = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntr || T >= row) return;
-    const float *tr = traces + i * (int64_t)ns;
     float v = 0.0f;
     if (T < ns) {
         float x[4];
         const int b = T >= 2 ? T : 0;
+        const float sc = SRC == KTM_TRACE_F32 ? 1.0f : geom[i].scale;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int t = b + k;
-            x[k] = t <= ns + 1 ? tr[t < ns ? t : ns - 1] * w[t] : 0.0f;
+            const int tt = t < ns ? t : ns - 1;
+            float tr;
+            if (SRC == KTM_TRACE_F32) {
+                tr = reinterpret_cast<const float *>(src)[i * (int64_t)ns + tt];
+            } else {
+                const uint16_t hq = reinterpret_cast<const uint16_t *>(src)[i * row + tt];
+                tr = sc * (SRC == KTM_TRACE_BF16 ? bf16_to_f32_(hq) : f16_to_f32_(hq));
+            }
+            x[k] = t <= ns + 1 ? tr * w[t] : 0.0f;
         }
         v = closed_tap(T, x[0], x[1], x[2], x[3]);
     }
@@ -202,65 +226,99 @@ __device__ __forceinline__ float f16_to_f32(uint16_t u)
     return (float)h;
 }
 
-/* One wave per trace: scale = max |tr| (1 for an all-zero trace), stored
- * value = tr * (1/scale) rounded to BF16 or FP16. */
+/* One lane per trace: scale = max |tr| (1 for an all-zero trace); the trace
+ * divided by scale is rounded to BF16 or FP16 with first-order error
+ * feedback (the rounding error of sample t is carried into sample t+1), which
+ * moves the quantisation noise towards the Nyquist frequency, where the
+ * [1,2,1] anti-alias taps of the migration have their zero. */
 template <int DT>
 __global__ void k_quantize(const float *__restrict__ traces, int64_t ntr, int ns, int64_t row,
                            uint16_t *__restrict__ q, ktm_trace_geom *__restrict__ geom)
 {
-    const int64_t i = (int64_t)blockIdx.x * (blockDim.x / KTM_WAVE) + (threadIdx.x / KTM_WAVE);
-    const int lane = threadIdx.x & (KTM_WAVE - 1);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntr) return;
     const float *tr = traces + i * (int64_t)ns;
     float m = 0.0f;
-    for (int t = lane; t < ns; t += KTM_WAVE) m = fmaxf(m, fabsf(tr[t]));
-#pragma unroll
-    for (int s = 32; s >= 1; s >>= 1) m = fmaxf(m, __shfl_xor(m, s, KTM_WAVE));
+    for (int t = 0; t < ns; ++t) m = fmaxf(m, fabsf(tr[t]));
     const float scale = m > 0.0f ? m : 1.0f;
     const float inv = 1.0f / scale;
     uint16_t *o = q + i * row;
-    for (int64_t t = lane; t < row; t += KTM_WAVE) {
-        const float v = t < ns ? tr[t] * inv : 0.0f;
-        o[t] = DT == KTM_TRACE_BF16 ? f32_to_bf16(v) : f32_to_f16(v);
+    float e = 0.0f;
+    for (int t = 0; t < ns; ++t) {
+        const float v = tr[t] * inv + e;
+        const uint16_t h = DT == KTM_TRACE_BF16 ? f32_to_bf16(v) : f32_to_f16(v);
+        o[t] = h;
+        e = v - (DT == KTM_TRACE_BF16 ? bf16_to_f32(h) : f16_to_f32(h));
     }
-    if (lane == 0) geom[i].scale = scale;
+    for (int64_t t = ns; t < row; ++t) o[t] = 0;
+    geom[i].scale = scale;
 }
 
 /* ------------------------------------------------------------------------ */
 /* migration                                                                 */
 /* ------------------------------------------------------------------------ */
 struct MigArgs {
-    const void *prep;                /* [ntr][row] float, or uint16 for BF16/FP16 */
+    const float *prep;               /* [ntr][row] filtered traces F(T) */
     const ktm_trace_geom *geom;
     int64_t ntr, row;
     float *km, *fold, *comp;
     const float *vel2;
     const int32_t *t1, *velx, *vely;
-    const float *t2, *colx, *coly, *slab_cx, *slab_cy, *w;
-    const double *slab_thr;
+    const float *t2, *colx, *coly, *slab_cx, *slab_cy, *slab_x2thr;
     unsigned long long *counter;
-    double aperture;
-    float dt, rdt;
+    float ap_x2thr;                  /* aperture gate on the squared CMP distance */
+    float aperture;                  /* for the conservative tile test only */
+    float dt, rdt;                   /* dt and RN(1/dt) */
     int ns, nh, nxp, nyp, nx, ny, jobs_y, nvel_xl, nsm;
     int tiles_x, ntiles, tx_cols, ty_cols;
     int rx0, ry0, rx1, ry1;          /* migrated region [rx0,rx1) x [ry0,ry1) */
 };
 
+/* RN(a/b) from rb = RN(1/b): q = RN(a rb) is within one ulp of a/b, the
+ * residual a - b q is exact under FMA, and one correction step rounds
+ * correctly (Markstein).  Checked against true division on 5e8 pairs,
+ * exhaustively over [2^-10, 32) for b = 0.004 (the DSR / dt division). */
+__device__ __forceinline__ float div_cr(float a, float b, float rb)
+{
+    const float q = a * rb;
+    const float r = __builtin_fmaf(-q, b, a);
+    return __builtin_fmaf(r, rb, q);
+}
+
+/* Correctly rounded double square root of 0 or a positive normal float:
+ * reciprocal square-root seed and the Goldschmidt/Newton correction used by
+ * the AMDGPU f64 sqrt expansion, without its denormal scaling (inputs are
+ * >= 2^-149 or exactly 0). */
+__device__ __forceinline__ double sqrt_cr(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    return x == 0.0 ? 0.0 : g;
+}
+
 /* Reference semantics of one traveltime index (Kirchhoff.c:1381-1384):
  * FP32 true divisions and sums, the two square roots in double and summed in
  * double, rounded to float, FP32 division by dt, floor.  Returns ns when the
- * sample falls past the trace (the T<ns test of Kirchhoff.c:1387). */
-__device__ __forceinline__ int exact_T(float dIS2, float dIR2, float t2, float v2, float dt, int ns)
+ * sample falls past the trace (the T<ns test of Kirchhoff.c:1387).  r and
+ * rdt are RN(1/v2) and RN(1/dt). */
+__device__ __forceinline__ int exact_T(float dIS2, float dIR2, float t2, float v2, float r,
+                                       float dt, float rdt, int ns)
 {
-    const float s1 = t2 + dIS2 / v2;
-    const float s2 = t2 + dIR2 / v2;
-    const float dsr = (float)(__builtin_sqrt((double)s1) + __builtin_sqrt((double)s2));
-    const float qq = dsr / dt;
+    const float s1 = t2 + div_cr(dIS2, v2, r);
+    const float s2 = t2 + div_cr(dIR2, v2, r);
+    const float dsr = (float)(sqrt_cr((double)s1) + sqrt_cr((double)s2));
+    const float qq = div_cr(dsr, dt, rdt);
     return qq < (float)ns ? (int)qq : ns;
 }
 
-/* Fast path of the traveltime index.  With r = RN(1/v2) and rdt = RN(1/dt)
- * precomputed,
+/* Fast path of the traveltime index.  With r = RN(1/v2) and rdt = RN(1/dt),
  *   q = RN( RN( sqrt_hw(fma(dIS2, r, t2)) + sqrt_hw(fma(dIR2, r, t2)) ) * rdt ).
  * Against the reference value Q = RN(dsr/dt) of exact_T (u = 2^-24):
  *   fma vs RN(t2 + RN(dIS2/v2))            relative 4u on the radicand
@@ -294,11 +352,18 @@ __device__ __forceinline__ int fast_T(float q, float nsf)
     return (int)__builtin_amdgcn_fmed3f(q, 0.0f, nsf);
 }
 
-/* The plain-FP32 variants fit 128 VGPRs, i.e. 4 waves per SIMD: two
- * 8-wave workgroups per CU, so one stages traces while the other computes. */
-template <int K, int W, int B, bool KAHAN, int SRC>
-__global__ void __launch_bounds__(8 * KTM_WAVE)
-__attribute__((amdgpu_waves_per_eu(KAHAN ? 1 : 4, KAHAN ? 4 : 4)))
+/* One workgroup = 8 waves = NCOL image columns x W time segments.  Each lane
+ * owns K samples t = seg*64K + 64k + lane of its column: the image values
+ * (km, fold count, Kahan compensation), t2[t] and 1/v^2 stay in registers
+ * for the whole launch, v^2 in LDS.  Traces stream through LDS in trace
+ * order: a chunk of 512 is filtered to those whose CMP can reach the tile
+ * (ordered compaction), then staged 8 at a time; per staged trace the
+ * aperture and slab gates of every column are evaluated lane-parallel
+ * (one lane per trace) and the passing traces are walked in order.
+ * The plain variants fit 128 VGPRs (4 waves per SIMD, two workgroups per CU,
+ * one staging while the other computes). */
+template <int K, int W, int B, bool KAHAN>
+__global__ void __launch_bounds__(8 * KTM_WAVE) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_migrate(const MigArgs a)
 {
     constexpr int NCOL = 8 / W;             /* image columns per workgroup */
@@ -311,7 +376,6 @@ k_migrate(const MigArgs a)
     int *sSel = reinterpret_cast<int *>(sG + B * 8);        /* [NT]     */
     int *sCnt = sSel + NT;                                  /* [NW]     */
     float *sV2 = reinterpret_cast<float *>(sCnt + NW);      /* [NCOL][W*K*64] v^2 per sample */
-    float *sW = sV2 + NCOL * W * K * KTM_WAVE;              /* [ns+2] (BF16/F16 only) */
 
     const int tid = threadIdx.x, lane = tid & (KTM_WAVE - 1), wv = tid / KTM_WAVE;
     /* XCD-aware tile order: consecutive workgroup ids go to different XCDs,
@@ -329,7 +393,7 @@ k_migrate(const MigArgs a)
     const float IX = a.colx[Xc], IY = a.coly[Yc];
     const int job = (Xc / a.nx) * a.jobs_y + (Yc / a.ny);
     const float scx = a.slab_cx[job], scy = a.slab_cy[job];
-    const double sthr = a.slab_thr[job];
+    const float sthr = a.slab_x2thr[job];
     const int ns = a.ns;
     const float nsf = (float)ns, nsf1 = (float)(ns + 1);
 
@@ -340,18 +404,15 @@ k_migrate(const MigArgs a)
     const float ccx = 0.5f * (a.colx[x0] + a.colx[x1]), ccy = 0.5f * (a.coly[y0] + a.coly[y1]);
     const float hx = 0.5f * fabsf(a.colx[x1] - a.colx[x0]);
     const float hy = 0.5f * fabsf(a.coly[y1] - a.coly[y0]);
-    const float reach = (float)a.aperture * 1.0001f + sqrtf(hx * hx + hy * hy) + 1.0f;
+    const float reach = a.aperture * 1.0001f + sqrtf(hx * hx + hy * hy) + 1.0f;
     const float reach2 = reach * reach;
 
-    if (SRC != KTM_TRACE_F32)
-        for (int t = tid; t < ns + 2; t += NT) sW[t] = a.w[t];
 
-    /* per-sample constants of this wave's column segment */
-    const int64_t pix = (((int64_t)Xc * a.nyp + Yc) * a.nh + h) * ns;
-    const int64_t vbase = ((int64_t)a.velx[Xc] * a.nvel_xl + a.vely[Yc]) * a.nsm;
     /* v^2 of this column for the exact path lives in LDS; the fast path keeps
      * 1/v^2 and t2 in registers */
     float *sV2c = sV2 + cw * (W * K * KTM_WAVE);
+    const int64_t pix = (((int64_t)Xc * a.nyp + Yc) * a.nh + h) * ns;
+    const int64_t vbase = ((int64_t)a.velx[Xc] * a.nvel_xl + a.vely[Yc]) * a.nsm;
     float r[K], t2v[K], acc[K], cmp[K];
     int cnt[K];
 #pragma unroll
@@ -404,40 +465,9 @@ k_migrate(const MigArgs a)
             /* ---- stage nb traces (filtered samples + geometry) into LDS ---- */
             for (int j = 0; j < nb; ++j) {
                 const int64_t ti = c0 + sSel[b0 + j];
-                if (SRC == KTM_TRACE_F32) {
-                    const float4 *src = reinterpret_cast<const float4 *>(
-                        reinterpret_cast<const float *>(a.prep) + ti * a.row);
-                    for (int c = tid; c < n4; c += NT)
-                        reinterpret_cast<float4 *>(sF + j * ldl)[c] = src[c];
-                } else {
-                    const uint16_t *src = reinterpret_cast<const uint16_t *>(a.prep) + ti * a.row;
-                    const float sc = a.geom[ti].scale;
-                    for (int c = tid; c < n4; c += NT) {
-                        const int T0 = 4 * c;
-                        float x[8];
-#pragma unroll
-                        for (int k = 0; k < 8; ++k) {
-                            const int t = T0 + k;
-                            const int tt = t < ns ? t : ns - 1;
-                            const float sv = SRC == KTM_TRACE_BF16 ? bf16_to_f32(src[tt])
-                                                                   : f16_to_f32(src[tt]);
-                            x[k] = t <= ns + 1 ? (sc * sv) * sW[t] : 0.0f;
-                        }
-                        float4 o;
-                        float *op = reinterpret_cast<float *>(&o);
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            const int T = T0 + u;
-                            float v = 0.0f;
-                            if (T < ns) {
-                                if (T >= 2) v = closed_tap(T, x[u], x[u + 1], x[u + 2], 0.0f);
-                                else v = closed_tap(T, x[0], x[1], x[2], x[3]);
-                            }
-                            op[u] = v;
-                        }
-                        reinterpret_cast<float4 *>(sF + j * ldl)[c] = o;
-                    }
-                }
+                const float4 *src = reinterpret_cast<const float4 *>(a.prep + ti * a.row);
+                for (int c = tid; c < n4; c += NT)
+                    reinterpret_cast<float4 *>(sF + j * ldl)[c] = src[c];
             }
             if (tid < nb * 2) {
                 const int j = tid >> 1, half = tid & 1;
@@ -448,37 +478,42 @@ k_migrate(const MigArgs a)
             __syncthreads();
 
             if (col_ok) {
-                for (int j = 0; j < nb; ++j) {
-                    const float *g = sG + j * 8;
-                    const float sxt = g[0], syt = g[1], gxt = g[2], gyt = g[3];
-                    const float cmx = g[4], cmy = g[5];
-                    /* slab prefilter, Kirchhoff.c:1334-1338 */
-                    const float ex = scx - cmx, ey = scy - cmy;
-                    const float ds = (float)__builtin_sqrt((double)(ex * ex + ey * ey));
-                    /* column gates and distances, Kirchhoff.c:1350-1354 */
-                    const float ax = IX - sxt, ay = IY - syt, bx = IX - gxt, by = IY - gyt;
-                    const float cx = IX - cmx, cy = IY - cmy;
-                    const float dIS2 = ax * ax + ay * ay;
-                    const float dIR2 = bx * bx + by * by;
-                    const float dc = (float)__builtin_sqrt((double)(cx * cx + cy * cy));
-                    const bool pass = (double)ds < sthr && (double)dc < a.aperture;
-                    if (!__builtin_amdgcn_readfirstlane((int)pass)) continue;
-                    ++npass;
+                /* gates and distances of Kirchhoff.c:1334-1354 for every staged
+                 * trace at once (lane j = trace j), on squared distances with
+                 * the exact thresholds of ktm_gate_threshold */
+                float g_is = 0.0f, g_ir = 0.0f;
+                bool gp = false;
+                if (lane < nb) {
+                    const float4 g0 = reinterpret_cast<const float4 *>(sG + lane * 8)[0];
+                    const float4 g1 = reinterpret_cast<const float4 *>(sG + lane * 8)[1];
+                    const float ax = IX - g0.x, ay = IY - g0.y, bx = IX - g0.z, by = IY - g0.w;
+                    const float cx = IX - g1.x, cy = IY - g1.y;
+                    const float ex = scx - g1.x, ey = scy - g1.y;
+                    g_is = ax * ax + ay * ay;
+                    g_ir = bx * bx + by * by;
+                    gp = ex * ex + ey * ey < sthr && cx * cx + cy * cy < a.ap_x2thr;
+                }
+                uint64_t pm = __ballot(gp);
+                npass += __popcll(pm);
+                while (pm) {
+                    const int j = __builtin_ctzll(pm);   /* traces in order */
+                    pm &= pm - 1ull;
+                    const float dIS2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g_is), j));
+                    const float dIR2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g_ir), j));
                     const float *Fr = sF + j * ldl;
-                    /* Hot loop, Kirchhoff.c:1374-1399.  Fast path: q = dsr/dt from
-                     * FMA (one reciprocal of v^2 per sample, precomputed), the
-                     * hardware square root and a reciprocal of dt.  Its distance
-                     * to the reference value is at most KTM_FAST_EPS*q (+1e-6);
-                     * whenever an integer lies within that distance the lane
-                     * recomputes T with the reference arithmetic, so T is
-                     * always the reference's. */
+                    /* Hot loop, Kirchhoff.c:1374-1399.  Fast path (fast_q) with
+                     * the reference arithmetic (exact_T) wherever an integer is
+                     * within the fast path's error bound of q, so T is always
+                     * the reference's. */
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
-                        const float q = fast_q(dIS2, dIR2, r[k], t2v[k], a.rdt);
+                        const float t2k = t2v[k];
+                        const float q = fast_q(dIS2, dIR2, r[k], t2k, a.rdt);
                         int T = fast_T(q, nsf);
                         if (fast_ambiguous(q, nsf1))
-                            T = exact_T(dIS2, dIR2, t2v[k],
-                                        sV2c[seg * K * KTM_WAVE + k * KTM_WAVE + lane], a.dt, ns);
+                            T = exact_T(dIS2, dIR2, t2k,
+                                        sV2c[seg * K * KTM_WAVE + k * KTM_WAVE + lane], r[k], a.dt,
+                                        a.rdt, ns);
                         const float f = Fr[T];
                         if (KAHAN) {
                             const float y = f - cmp[k];
@@ -702,13 +737,16 @@ __global__ void k_selftest_traveltime(const float4 *__restrict__ in, int64_t n, 
     const float q = fast_q(v.x, v.y, r, v.z, rdt);
     const float nsf = (float)ns, nsf1 = (float)(ns + 1);
     int T = fast_T(q, nsf);
-    if (fast_ambiguous(q, nsf1)) T = exact_T(v.x, v.y, v.z, v.w, dt, ns);
+    if (fast_ambiguous(q, nsf1)) T = exact_T(v.x, v.y, v.z, v.w, r, dt, rdt, ns);
+    /* the reference computed independently of exact_T: the compiler's
+     * correctly rounded division and double square root */
     const float s1 = v.z + v.x / v.w, s2 = v.z + v.y / v.w;
     const float dsr = (float)(__builtin_sqrt((double)s1) + __builtin_sqrt((double)s2));
+    const float qq = dsr / dt;
     q_fast[i] = q;
-    q_ref[i] = dsr / dt;
+    q_ref[i] = qq;
     t_kernel[i] = T;
-    t_ref[i] = exact_T(v.x, v.y, v.z, v.w, dt, ns);
+    t_ref[i] = qq < nsf ? (int)qq : ns;
 }
 
 extern "C" int ktm_selftest_sqrt(const float *x, int64_t n, float *out_hw, void *stream)
@@ -750,7 +788,8 @@ extern "C" void ktm_plan_destroy(ktm_plan *p)
 {
     if (!p) return;
     void *bufs[] = {p->d_t2, p->d_t1, p->d_colx, p->d_coly, p->d_velx, p->d_vely,
-                    p->d_slab_cx, p->d_slab_cy, p->d_slab_thr, p->d_vel2, p->d_w, p->d_hconv};
+                    p->d_slab_cx, p->d_slab_cy, p->d_slab_x2thr, p->d_vel2, p->d_w, p->d_hconv,
+                    p->d_stage};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     ktm_tables_free(&p->tb);
@@ -795,7 +834,7 @@ extern "C" int ktm_plan_create(ktm_plan **out, const ktm_config *cfg, int ns, fl
     if (st == KTM_OK) st = upload(&p->d_vely, tb->vely, (size_t)tb->s.nyp);
     if (st == KTM_OK) st = upload(&p->d_slab_cx, tb->slab_cx, (size_t)tb->s.no_job);
     if (st == KTM_OK) st = upload(&p->d_slab_cy, tb->slab_cy, (size_t)tb->s.no_job);
-    if (st == KTM_OK) st = upload(&p->d_slab_thr, tb->slab_thr, (size_t)tb->s.no_job);
+    if (st == KTM_OK) st = upload(&p->d_slab_x2thr, tb->slab_x2thr, (size_t)tb->s.no_job);
     if (st == KTM_OK) st = upload(&p->d_w, tb->w, (size_t)ns + 2);
     if (st == KTM_OK) st = upload(&p->d_hconv, tb->hconv, (size_t)ns);
     if (st != KTM_OK) { ktm_plan_destroy(p); return st; }
@@ -850,8 +889,8 @@ extern "C" int ktm_prep(const ktm_plan *p, const float *traces, const int32_t *c
     if (prep_mode != KTM_PREP_REFERENCE && prep_mode != KTM_PREP_CLOSED) return KTM_ERR_ARG;
     if (dtype != KTM_TRACE_F32 && dtype != KTM_TRACE_BF16 && dtype != KTM_TRACE_F16)
         return KTM_ERR_ARG;
-    /* reduced-precision storage keeps the raw trace; its filter is the
-     * closed form evaluated while staging */
+    /* reduced-precision storage keeps the raw trace; its filter (the closed
+     * form) is applied when a block is migrated */
     if (dtype != KTM_TRACE_F32 && prep_mode != KTM_PREP_CLOSED) return KTM_ERR_UNSUPPORTED;
     hipStream_t s = (hipStream_t)stream;
     int st = ktm_geometry(p, coords, ntr, geom, stream);
@@ -867,14 +906,15 @@ extern "C" int ktm_prep(const ktm_plan *p, const float *traces, const int32_t *c
             const int nt = 256;
             for (int64_t i0 = 0; i0 < ntr; i0 += 65535) {
                 const int64_t n = ntr - i0 < 65535 ? ntr - i0 : 65535;
-                hipLaunchKernelGGL(k_prep_closed, dim3((unsigned)((p->row + nt - 1) / nt), (unsigned)n),
-                                   dim3(nt), 0, s, traces + i0 * ns, n, ns, p->row, p->d_w,
-                                   (float *)prepared + i0 * p->row);
+                hipLaunchKernelGGL(k_prep_closed<KTM_TRACE_F32>,
+                                   dim3((unsigned)((p->row + nt - 1) / nt), (unsigned)n), dim3(nt),
+                                   0, s, (const void *)(traces + i0 * ns), geom + i0, n, ns,
+                                   p->row, p->d_w, (float *)prepared + i0 * p->row);
             }
         }
     } else {
-        const int nt = 256, tpb = nt / KTM_WAVE;
-        const dim3 grid((unsigned)((ntr + tpb - 1) / tpb));
+        const int nt = 64;
+        const dim3 grid((unsigned)((ntr + nt - 1) / nt));
         if (dtype == KTM_TRACE_BF16)
             hipLaunchKernelGGL(k_quantize<KTM_TRACE_BF16>, grid, dim3(nt), 0, s, traces, ntr, ns,
                                p->row, (uint16_t *)prepared, geom);
@@ -898,26 +938,56 @@ extern "C" int ktm_image_reset(const ktm_plan *p, float *km, float *fold, float 
 /* kernel choice: K sample slots per lane and W waves per column cover ns */
 typedef void (*mig_fn)(const MigArgs);
 
+/* traces staged per batch: 8 with plain sums; 6 with the Kahan image, whose
+ * extra registers leave less room (two workgroups per CU either way) */
 template <int K, int W>
-static mig_fn pick_src(int dtype, int accum)
+static mig_fn pick_accum(int accum, int *B)
 {
-    if (dtype == KTM_TRACE_F32)
-        return accum == KTM_ACCUM_KAHAN ? k_migrate<K, W, 8, true, KTM_TRACE_F32>
-                                        : k_migrate<K, W, 8, false, KTM_TRACE_F32>;
-    if (dtype == KTM_TRACE_BF16) return k_migrate<K, W, 8, true, KTM_TRACE_BF16>;
-    return k_migrate<K, W, 8, true, KTM_TRACE_F16>;
+    *B = accum == KTM_ACCUM_PLAIN ? 8 : 6;
+    return accum == KTM_ACCUM_KAHAN ? k_migrate<K, W, 6, true> : k_migrate<K, W, 8, false>;
 }
 
 /* W waves share one image column (each owns K*64 consecutive samples);
  * a workgroup is 8 waves, so it covers 8/W columns. */
-static mig_fn pick_migrate(int ns, int dtype, int accum, int *W, int *K)
+static mig_fn pick_migrate(int ns, int accum, int *W, int *K, int *B)
 {
-    if (ns <= 512) { *W = 1; *K = 8; return pick_src<8, 1>(dtype, accum); }
-    if (ns <= 1024) { *W = 2; *K = 8; return pick_src<8, 2>(dtype, accum); }
-    if (ns <= 1536) { *W = 2; *K = 12; return pick_src<12, 2>(dtype, accum); }
-    if (ns <= 2048) { *W = 4; *K = 8; return pick_src<8, 4>(dtype, accum); }
+    if (ns <= 512) { *W = 1; *K = 8; return pick_accum<8, 1>(accum, B); }
+    if (ns <= 1024) { *W = 2; *K = 8; return pick_accum<8, 2>(accum, B); }
+    if (ns <= 1536) { *W = 2; *K = 12; return pick_accum<12, 2>(accum, B); }
+    if (ns <= 2048) { *W = 4; *K = 8; return pick_accum<8, 4>(accum, B); }
     *W = 4; *K = 12;
-    return pick_src<12, 4>(dtype, accum);
+    return pick_accum<12, 4>(accum, B);
+}
+
+/* FP32 filtered traces of a BF16/FP16 block, in a workspace the plan owns and
+ * grows on demand (the only allocation on the migration path) */
+static int decode_block(const ktm_plan *cp, const void *prepared, int dtype,
+                        const ktm_trace_geom *geom, int64_t ntr, hipStream_t s, const float **out)
+{
+    ktm_plan *p = const_cast<ktm_plan *>(cp);
+    const size_t need = (size_t)ntr * (size_t)p->row;
+    if (need > p->stage_cap) {
+        if (p->d_stage) HIP_TRY(hipFree(p->d_stage));
+        p->d_stage = nullptr;
+        p->stage_cap = 0;
+        if (hipMalloc((void **)&p->d_stage, need * sizeof(float)) != hipSuccess) return KTM_ERR_NOMEM;
+        p->stage_cap = need;
+    }
+    const int nt = 256;
+    for (int64_t i0 = 0; i0 < ntr; i0 += 65535) {
+        const int64_t n = ntr - i0 < 65535 ? ntr - i0 : 65535;
+        const dim3 grid((unsigned)((p->row + nt - 1) / nt), (unsigned)n);
+        const void *src = reinterpret_cast<const uint16_t *>(prepared) + i0 * p->row;
+        if (dtype == KTM_TRACE_BF16)
+            hipLaunchKernelGGL(k_prep_closed<KTM_TRACE_BF16>, grid, dim3(nt), 0, s, src, geom + i0, n,
+                               p->tb.s.ns, p->row, p->d_w, p->d_stage + i0 * p->row);
+        else
+            hipLaunchKernelGGL(k_prep_closed<KTM_TRACE_F16>, grid, dim3(nt), 0, s, src, geom + i0, n,
+                               p->tb.s.ns, p->row, p->d_w, p->d_stage + i0 * p->row);
+    }
+    HIP_TRY(hipGetLastError());
+    *out = p->d_stage;
+    return KTM_OK;
 }
 
 extern "C" int ktm_migrate_block(const ktm_plan *p, const void *prepared, int dtype, int prep_mode,
@@ -930,32 +1000,38 @@ extern "C" int ktm_migrate_block(const ktm_plan *p, const void *prepared, int dt
     if (!prepared || !geom) return KTM_ERR_ARG;
     if (accum != KTM_ACCUM_PLAIN && accum != KTM_ACCUM_KAHAN) return KTM_ERR_ARG;
     if (accum == KTM_ACCUM_KAHAN && !comp) return KTM_ERR_ARG;
-    if (dtype != KTM_TRACE_F32 && (accum != KTM_ACCUM_KAHAN || prep_mode != KTM_PREP_CLOSED))
-        return KTM_ERR_UNSUPPORTED;
-    (void)prep_mode;
+    if (dtype != KTM_TRACE_F32 && dtype != KTM_TRACE_BF16 && dtype != KTM_TRACE_F16) return KTM_ERR_ARG;
+    if (dtype != KTM_TRACE_F32 && prep_mode != KTM_PREP_CLOSED) return KTM_ERR_UNSUPPORTED;
     const ktm_setup *s = &p->tb.s;
+    if (job >= s->no_job) return KTM_ERR_ARG;
+    hipStream_t hs = (hipStream_t)stream;
+    const float *F = reinterpret_cast<const float *>(prepared);
+    if (dtype != KTM_TRACE_F32) {
+        const int st = decode_block(p, prepared, dtype, geom, ntr, hs, &F);
+        if (st != KTM_OK) return st;
+    }
     MigArgs a;
     memset(&a, 0, sizeof a);
-    a.prep = prepared; a.geom = geom; a.ntr = ntr; a.row = p->row;
+    a.prep = F; a.geom = geom; a.ntr = ntr; a.row = p->row;
     a.km = km; a.fold = fold; a.comp = comp;
     a.vel2 = p->d_vel2; a.t1 = p->d_t1; a.velx = p->d_velx; a.vely = p->d_vely;
     a.t2 = p->d_t2; a.colx = p->d_colx; a.coly = p->d_coly;
-    a.slab_cx = p->d_slab_cx; a.slab_cy = p->d_slab_cy; a.slab_thr = p->d_slab_thr; a.w = p->d_w;
+    a.slab_cx = p->d_slab_cx; a.slab_cy = p->d_slab_cy; a.slab_x2thr = p->d_slab_x2thr;
     a.counter = (unsigned long long *)counter;
-    a.aperture = p->cfg.aperture;
+    a.ap_x2thr = p->tb.ap_x2thr;
+    a.aperture = (float)p->cfg.aperture;
     a.dt = s->dt;
     a.rdt = 1.0f / s->dt;
     a.ns = s->ns; a.nh = s->nh; a.nxp = s->nxp; a.nyp = s->nyp; a.nx = s->nx; a.ny = s->ny;
     a.jobs_y = p->cfg.jobs_y; a.nvel_xl = p->cfg.nvel_xlines; a.nsm = s->nsm;
-    if (job >= s->no_job) return KTM_ERR_ARG;
     if (job < 0) {
         a.rx0 = 0; a.ry0 = 0; a.rx1 = s->nxp; a.ry1 = s->nyp;
     } else { /* slab of Kirchhoff.c:1341-1347, job -> slice as in 287-295 */
         a.rx0 = (job / p->cfg.jobs_y) * s->nx; a.rx1 = a.rx0 + s->nx;
         a.ry0 = (job % p->cfg.jobs_y) * s->ny; a.ry1 = a.ry0 + s->ny;
     }
-    int W = 1, K = 8;
-    mig_fn fn = pick_migrate(s->ns, dtype, accum, &W, &K);
+    int W = 1, K = 8, B = 8;
+    mig_fn fn = pick_migrate(s->ns, accum, &W, &K, &B);
     /* tile of 8/W columns: a row on a 2-D line, as square as possible in 3-D */
     const int ncol = 8 / W;
     a.tx_cols = s->nyp == 1 ? ncol : (ncol == 8 ? 4 : ncol == 4 ? 2 : ncol);
@@ -964,14 +1040,13 @@ extern "C" int ktm_migrate_block(const ktm_plan *p, const void *prepared, int dt
     const int tiles_y = (a.ry1 - a.ry0 + a.ty_cols - 1) / a.ty_cols;
     a.ntiles = a.tiles_x * tiles_y;
     const int nthreads = 8 * KTM_WAVE;
-    const size_t lds = sizeof(float) * (size_t)(8 * p->row + 8 * 8) + sizeof(int) * (size_t)(nthreads + 8) +
-                       sizeof(float) * (size_t)(8 * K * KTM_WAVE) +
-                       (dtype != KTM_TRACE_F32 ? sizeof(float) * (size_t)(s->ns + 2) : 0);
+    const size_t lds = sizeof(float) * (size_t)(B * p->row + B * 8) +
+                       sizeof(int) * (size_t)(nthreads + 8) + sizeof(float) * (size_t)(8 * K * KTM_WAVE);
     if (lds > 160 * 1024) return KTM_ERR_UNSUPPORTED;
     const int per = (a.ntiles + 7) / 8;
     const dim3 grid((unsigned)(per * 8), (unsigned)s->nh);
     HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(fn, grid, dim3(nthreads), lds, (hipStream_t)stream, a);
+    hipLaunchKernelGGL(fn, grid, dim3(nthreads), lds, hs, a);
     HIP_TRY(hipGetLastError());
     return KTM_OK;
 }

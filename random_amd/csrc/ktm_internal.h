@@ -25,6 +25,8 @@ typedef struct ktm_tables {
     float   *slab_cx;     /* [no_job] slab centre IX, Kirchhoff.c:1332                */
     float   *slab_cy;     /* [no_job] slab centre IY, Kirchhoff.c:1333                */
     double  *slab_thr;    /* [no_job] aperture + add_on, Kirchhoff.c:1335-1338        */
+    float   *slab_x2thr;  /* [no_job] the same gate on the squared distance (exact)  */
+    float    ap_x2thr;    /* aperture gate of Kirchhoff.c:1354 on the squared distance */
     float   *w;           /* [ns+2] CLOSED-mode sample weights: x[t] = tr[min(t,ns-1)]*w[t] */
     double  *hconv;       /* [ns]   circulant post-stack kernel, real IDFT of anti_alias / ns */
 } ktm_tables;

@@ -72,6 +72,7 @@ int ktm_setup_init(const ktm_config *c, int ns, float dt, int nsm, float dtm,
 {
     if (!c || !s || ns <= 0 || nsm <= 0 || !(dt > 0.0f) || !(dtm > 0.0f)) return KTM_ERR_ARG;
     if (c->jobs_x <= 0 || c->jobs_y <= 0 || c->nh <= 0 || !(c->dh != 0.0)) return KTM_ERR_ARG;
+    if (!(c->aperture >= 1e-3) || !(c->aperture < 1e30)) return KTM_ERR_ARG;
     memset(s, 0, sizeof *s);
     s->ns = ns; s->nsm = nsm; s->dt = dt; s->dtm = dtm;
     s->Nx = cells_along(c->start_iline, c->end_iline, c->inc_iline);
@@ -108,6 +109,25 @@ int ktm_vel_index(const ktm_config *c, int col, int axis)
     return k;
 }
 
+float ktm_gate_threshold(double A)
+{
+    /* The reference gates are (double)(float)sqrt((double)x) < A for a float x
+     * (Kirchhoff.c:1334-1338, 1352-1354).  The double rounding of sqrt to
+     * float is innocuous (53 >= 2*24+2), so the left side is RN_f(sqrt(x)).
+     * With B the largest float below A and M the midpoint of B and its
+     * successor, RN_f(sqrt(x)) <= B  <=>  sqrt(x) < M  <=>  x < M^2: a tie
+     * would need x == M^2, impossible since M has 25 significant bits and M^2
+     * needs at least 49.  M^2 is exact in double; the float threshold is the
+     * smallest float above it. */
+    float B = (float)A;
+    if ((double)B >= A) B = nextafterf(B, -INFINITY);
+    const double M = 0.5 * ((double)B + (double)nextafterf(B, INFINITY));
+    const double M2 = M * M;
+    float thr = (float)M2;
+    if ((double)thr <= M2) thr = nextafterf(thr, INFINITY);
+    return thr;
+}
+
 /* image coordinate of column `col` (possibly fractional: the slab centre) */
 static float image_coord(const ktm_config *c, double col, int axis)
 {
@@ -123,7 +143,7 @@ void ktm_tables_free(ktm_tables *tb)
 {
     free(tb->anti_alias); free(tb->t2); free(tb->t1); free(tb->colx); free(tb->coly);
     free(tb->velx); free(tb->vely); free(tb->slab_cx); free(tb->slab_cy);
-    free(tb->slab_thr); free(tb->w); free(tb->hconv);
+    free(tb->slab_thr); free(tb->slab_x2thr); free(tb->w); free(tb->hconv);
     memset(tb, 0, sizeof *tb);
 }
 
@@ -144,10 +164,11 @@ int ktm_tables_build(const ktm_config *c, int ns, float dt, int nsm, float dtm,
     tb->slab_cx = (float *)malloc(sizeof(float) * (size_t)s->no_job);
     tb->slab_cy = (float *)malloc(sizeof(float) * (size_t)s->no_job);
     tb->slab_thr = (double *)malloc(sizeof(double) * (size_t)s->no_job);
+    tb->slab_x2thr = (float *)malloc(sizeof(float) * (size_t)s->no_job);
     tb->w = (float *)malloc(sizeof(float) * (size_t)(ns + 2));
     tb->hconv = (double *)malloc(sizeof(double) * (size_t)ns);
     if (!tb->anti_alias || !tb->t2 || !tb->t1 || !tb->colx || !tb->coly || !tb->velx ||
-        !tb->vely || !tb->slab_cx || !tb->slab_cy || !tb->slab_thr || !tb->w || !tb->hconv) {
+        !tb->vely || !tb->slab_cx || !tb->slab_cy || !tb->slab_thr || !tb->slab_x2thr || !tb->w || !tb->hconv) {
         ktm_tables_free(tb);
         return KTM_ERR_NOMEM;
     }
@@ -182,7 +203,9 @@ int ktm_tables_build(const ktm_config *c, int ns, float dt, int nsm, float dtm,
         tb->slab_cx[j] = image_coord(c, s->nx / 2.0 + xs * s->nx, 0);
         tb->slab_cy[j] = image_coord(c, s->ny / 2.0 + ys * s->ny, 1);
         tb->slab_thr[j] = c->aperture + add_on;
+        tb->slab_x2thr[j] = ktm_gate_threshold(tb->slab_thr[j]);
     }
+    tb->ap_x2thr = ktm_gate_threshold(c->aperture);
     /* CLOSED-mode weights: x[t] = tr[t] / (4 t2[t]) for 1 <= t < ns,
      * x[0] = tr[0] (Kirchhoff.c:1314), tail x[t] = tr[ns-1] / (dt^2 t^2)
      * for t >= ns (Kirchhoff.c:1319-1320). */

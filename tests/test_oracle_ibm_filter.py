@@ -113,3 +113,20 @@ def test_setup_decomposition_matches_oracle():
         assert info["nxp"] == g.nx * cfg.jobs_x and info["nyp"] == g.ny * cfg.jobs_y
         assert np.float32(info["sinA"]) == np.float32(g.sinA)
         assert np.float32(info["cosA"]) == np.float32(g.cosA)
+
+
+@pytest.mark.parametrize("A", [2000.0, 2500.0, 3000.0, 2000.3, 3075.125123456, 9571.06781186548,
+                               1.0e-2, 123456.789])
+def test_gate_threshold_reproduces_reference_gate(A):
+    """x < ktm_gate_threshold(A) must equal the reference gate
+    (double)(float)sqrt((double)x) < A (Kirchhoff.c:1334-1338,1352-1354) for
+    every float x; checked on the 2e5 floats around the threshold and on
+    random floats."""
+    thr = np.float32(_lib.lib().ktm_gate_threshold(A))
+    u = thr.view(np.int32)
+    x = (np.arange(-100000, 100000, dtype=np.int64) + int(u)).astype(np.int32).view(np.float32)
+    rng = np.random.default_rng(0)
+    x = np.concatenate([x, (rng.uniform(0, 4 * A * A, 200000)).astype(np.float32)])
+    ref = np.sqrt(x.astype(np.float64)).astype(np.float32).astype(np.float64) < A
+    got = x < thr
+    assert np.array_equal(got, ref)
