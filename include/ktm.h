@@ -175,6 +175,20 @@ int ktm_poststack(const ktm_plan *plan, const float *km, const float *fold,
                   float *out, void *stream);
 
 
+/* ---- adjoint pair (device), for least-squares migration and gradients ----
+ * With the CLOSED prep, migration is linear in the traces:
+ *   km = L^T d     (ktm_prep CLOSED + ktm_migrate_block into a zeroed km;
+ *                   fold is a by-product that depends on geometry only)
+ *   d  = L km      (ktm_demigrate: the exact transpose, same traveltime
+ *                   index, gates and filter, demigration / modelling)
+ * and the post-stack chain image = S km (ktm_poststack, S linear for a fixed
+ * fold) has the transpose km = S^T image (ktm_poststack_adjoint).  No
+ * counterpart in the reference, which only migrates. */
+int ktm_demigrate(const ktm_plan *plan, const float *km, const ktm_trace_geom *geom,
+                  int64_t ntr, float *traces, void *stream);
+int ktm_poststack_adjoint(const ktm_plan *plan, const float *image, const float *fold,
+                          float *km, void *stream);
+
 /* Host convenience: whole reference job loop for all jobs (Kirchhoff.c:
  * 347-350 on one rank): prep + migrate in blocks of cfg->ntblock + post-stack.
  * Host arrays; km/fold/image are [nxp][nyp][nh][ns] (any may be NULL). */

@@ -110,6 +110,8 @@ _PROTOS = {
     "ktm_synth_noise": (_i32, [_vp, _i64, _f32, C.c_uint64, _vp]),
     "ktm_selftest_sqrt": (_i32, [_vp, _i64, _vp, _vp]),
     "ktm_gate_threshold": (C.c_float, [C.c_double]),
+    "ktm_demigrate": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp]),
+    "ktm_poststack_adjoint": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "ktm_selftest_traveltime": (_i32, [_vp, _i64, _f32, _i32, _vp, _vp, _vp, _vp, _vp]),
 }
 

@@ -50,6 +50,7 @@ struct ktm_plan {
     double *d_hconv;        /* [ns]   post-stack kernel */
     float *d_stage;         /* FP32 filtered traces of a BF16/FP16 block */
     size_t stage_cap;       /* elements of d_stage */
+    float *d_pix_v2, *d_pix_r;  /* per-pixel v^2 and RN(1/v^2), built for demigration */
 };
 
 #define HIP_TRY(expr)                                  \
@@ -771,6 +772,187 @@ extern "C" int ktm_selftest_traveltime(const float *in4, int64_t n, float dt, in
     return KTM_OK;
 }
 
+
+/* ------------------------------------------------------------------------ */
+/* adjoint pair: demigration (L) and the post-stack transpose                */
+/* ------------------------------------------------------------------------ */
+/* Per-pixel velocity tables for trace-parallel kernels: v2 and RN(1/v2) of
+ * every (column, sample), the same values k_migrate keeps per lane. */
+__global__ void k_pixel_velocity(const float *__restrict__ vel2, const int32_t *__restrict__ velx,
+                                 const int32_t *__restrict__ vely, const int32_t *__restrict__ t1,
+                                 int nxp, int nyp, int ns, int nvel_xl, int nsm,
+                                 float *__restrict__ pv2, float *__restrict__ pr)
+{
+    const int64_t n = (int64_t)nxp * nyp * ns;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int t = (int)(e % ns);
+        const int64_t c = e / ns;
+        const int X = (int)(c / nyp), Y = (int)(c % nyp);
+        const float v2 = vel2[((int64_t)velx[X] * nvel_xl + vely[Y]) * nsm + t1[t]];
+        pv2[e] = v2;
+        pr[e] = 1.0f / v2;
+    }
+}
+
+struct DemigArgs {
+    const float *km;                 /* image [nxp][nyp][nh][ns] */
+    const ktm_trace_geom *geom;
+    int64_t ntr;
+    float *traces;                   /* out [ntr][ns] */
+    const float *pv2, *pr, *t2, *w, *colx, *coly, *slab_cx, *slab_cy, *slab_x2thr;
+    float ap_x2thr, aperture, dt, rdt;
+    int ns, nh, nxp, nyp, nx, ny, jobs_y;
+};
+
+/* L, the exact transpose of migration with the CLOSED filter (km = L^T d).
+ * One workgroup per trace: the waves walk the image columns of the box around
+ * the trace's aperture disc, gate them exactly like k_migrate, and for every
+ * sample t of a passing column with T(t) < ns add km[column][off][t] into
+ * g[T] (LDS atomics).  The trace is then the transpose of the 3-tap filter
+ * and of the sample weights applied to g. */
+template <int K, int W>
+__global__ void __launch_bounds__(8 * KTM_WAVE) k_demigrate(const DemigArgs a)
+{
+    constexpr int NCOL = 8 / W;
+    extern __shared__ float sg[];     /* [ns + 4] */
+    const int64_t i = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & (KTM_WAVE - 1), wv = tid / KTM_WAVE;
+    const int cw = wv / W, seg = wv % W;
+    const int ns = a.ns;
+    for (int t = tid; t < ns + 4; t += 8 * KTM_WAVE) sg[t] = 0.0f;
+    __syncthreads();
+    const ktm_trace_geom g = a.geom[i];
+    const float nsf = (float)ns, nsf1 = (float)(ns + 1);
+    if (g.off >= 0 && g.off < a.nh) {
+        /* columns whose centre can be within the aperture of the CMP */
+        int xlo = 0, xhi = a.nxp - 1, ylo = 0, yhi = a.nyp - 1;
+        if (a.nxp > 1) {
+            const float step = (a.colx[a.nxp - 1] - a.colx[0]) / (float)(a.nxp - 1);
+            if (step != 0.0f) {
+                const float xc = (g.cmpx - a.colx[0]) / step, half = a.aperture / fabsf(step) + 2.0f;
+                xlo = max(0, (int)floorf(xc - half));
+                xhi = min(a.nxp - 1, (int)ceilf(xc + half));
+            }
+        }
+        if (a.nyp > 1) {
+            const float step = (a.coly[a.nyp - 1] - a.coly[0]) / (float)(a.nyp - 1);
+            if (step != 0.0f) {
+                const float yc = (g.cmpy - a.coly[0]) / step, half = a.aperture / fabsf(step) + 2.0f;
+                ylo = max(0, (int)floorf(yc - half));
+                yhi = min(a.nyp - 1, (int)ceilf(yc + half));
+            }
+        }
+        const int nbx = xhi - xlo + 1, nby = yhi - ylo + 1;
+        const int64_t ncols = nbx > 0 && nby > 0 ? (int64_t)nbx * nby : 0;
+        for (int64_t cidx = cw; cidx < ncols; cidx += NCOL) {
+            const int X = xlo + (int)(cidx / nby), Y = ylo + (int)(cidx % nby);
+            const float IX = a.colx[X], IY = a.coly[Y];
+            const int job = (X / a.nx) * a.jobs_y + (Y / a.ny);
+            const float ex = a.slab_cx[job] - g.cmpx, ey = a.slab_cy[job] - g.cmpy;
+            const float cx = IX - g.cmpx, cy = IY - g.cmpy;
+            const bool pass = ex * ex + ey * ey < a.slab_x2thr[job] && cx * cx + cy * cy < a.ap_x2thr;
+            if (!__builtin_amdgcn_readfirstlane((int)pass)) continue;
+            const float ax = IX - g.sxt, ay = IY - g.syt, bx = IX - g.gxt, by = IY - g.gyt;
+            const float dIS2 = ax * ax + ay * ay, dIR2 = bx * bx + by * by;
+            const int64_t pixc = ((int64_t)X * a.nyp + Y) * ns;
+            const float *mk = a.km + (((int64_t)X * a.nyp + Y) * a.nh + g.off) * ns;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int t = seg * K * KTM_WAVE + k * KTM_WAVE + lane;
+                if (t < ns) {
+                    const float r = a.pr[pixc + t], t2 = a.t2[t];
+                    const float q = fast_q(dIS2, dIR2, r, t2, a.rdt);
+                    int T = fast_T(q, nsf);
+                    if (fast_ambiguous(q, nsf1))
+                        T = exact_T(dIS2, dIR2, t2, a.pv2[pixc + t], r, a.dt, a.rdt, ns);
+                    if (T < ns) atomicAdd(&sg[T], mk[t]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    /* transpose of F = closed_tap(x), x[t] = tr[min(t, ns-1)] * w[t] */
+    float *out = a.traces + i * (int64_t)ns;
+    for (int t = tid; t < ns; t += 8 * KTM_WAVE) {
+        float acc = 0.0f;
+        const int nt_ = t == ns - 1 ? 3 : 1;   /* tr[ns-1] also feeds x[ns], x[ns+1] */
+        for (int u = 0; u < nt_; ++u) {
+            const int tt = t + u;
+            float xb = 0.0f;
+            if (tt >= 2 && tt < ns) xb += sg[tt];
+            if (tt - 1 >= 2 && tt - 1 < ns) xb += 2.0f * sg[tt - 1];
+            if (tt - 2 >= 2 && tt - 2 < ns) xb += sg[tt - 2];
+            if (tt == 0) xb += 4.0f * sg[0] + 2.0f * sg[1];
+            if (tt == 1) xb += 4.0f * sg[0] + 2.0f * sg[1];
+            if (tt == 2) xb += 2.0f * sg[0] + 2.0f * sg[1];
+            if (tt == 3) xb += sg[1];
+            acc += a.w[tt] * xb;
+        }
+        out[t] = acc;
+    }
+}
+
+/* Transpose of the post-stack chain image = P0 H C D km of Kirchhoff.c:
+ * 1457-1479 (D: first difference with d[0] = 0, C: -dt^2 t / fold[t],
+ * H: the even circular low-pass, P0: out[0] = 0):  km_bar = D^T C H P0 img. */
+template <int SLOTS>
+__global__ void __launch_bounds__(PS_NT)
+k_poststack_adjoint(const float *__restrict__ img, const float *__restrict__ fold, int64_t ntraces,
+                    int ns, float dt, const double *__restrict__ hconv, float *__restrict__ out)
+{
+    extern __shared__ double lds_d[];
+    double *hh = lds_d;              /* [2*ns] */
+    double *zz = lds_d + 2 * ns;     /* [ns][PS_TR] P0 img */
+    double *uu = zz + ns * PS_TR;    /* [ns+1][PS_TR] C H P0 img */
+    const int64_t tr0 = (int64_t)blockIdx.x * PS_TR;
+    for (int m = threadIdx.x; m < 2 * ns; m += PS_NT) hh[m] = hconv[m < ns ? m : m - ns];
+    for (int e = threadIdx.x; e < ns * PS_TR; e += PS_NT) {
+        const int u = e / PS_TR, b = e - u * PS_TR;
+        const int64_t tr = tr0 + b;
+        zz[u * PS_TR + b] = tr < ntraces && u >= 1 ? (double)img[tr * ns + u] : 0.0;
+    }
+    __syncthreads();
+    double acc[SLOTS][PS_TR];
+#pragma unroll
+    for (int s = 0; s < SLOTS; ++s)
+#pragma unroll
+        for (int b = 0; b < PS_TR; ++b) acc[s][b] = 0.0;
+    for (int u = 0; u < ns; ++u) {
+        double zv[PS_TR];
+#pragma unroll
+        for (int b = 0; b < PS_TR; ++b) zv[b] = zz[u * PS_TR + b];
+#pragma unroll
+        for (int s = 0; s < SLOTS; ++s) {
+            const int t = s * PS_NT + threadIdx.x;
+            /* H is symmetric: (H^T z)[t] = sum_u h[(u - t) mod ns] z[u] = sum_u h[(t - u) mod ns] z[u] */
+            const double hv = t < ns ? hh[t - u + ns] : 0.0;
+#pragma unroll
+            for (int b = 0; b < PS_TR; ++b) acc[s][b] = __builtin_fma(hv, zv[b], acc[s][b]);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < SLOTS; ++s) {
+        const int t = s * PS_NT + threadIdx.x;
+        if (t < ns)
+#pragma unroll
+            for (int b = 0; b < PS_TR; ++b) {
+                const int64_t tr = tr0 + b;
+                double c = 0.0;
+                if (tr < ntraces && t >= 1)
+                    c = -(double)dt * (double)dt * (double)t / (double)fold[tr * ns + t];
+                uu[t * PS_TR + b] = c * acc[s][b];
+            }
+    }
+    for (int b = threadIdx.x; b < PS_TR; b += PS_NT) uu[ns * PS_TR + b] = 0.0;
+    __syncthreads();
+    for (int e = threadIdx.x; e < ns * PS_TR; e += PS_NT) {
+        const int t = e / PS_TR, b = e - t * PS_TR;
+        const int64_t tr = tr0 + b;
+        if (tr < ntraces) out[tr * ns + t] = (float)(uu[t * PS_TR + b] - uu[(t + 1) * PS_TR + b]);
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* host glue                                                                 */
 /* ------------------------------------------------------------------------ */
@@ -789,7 +971,7 @@ extern "C" void ktm_plan_destroy(ktm_plan *p)
     if (!p) return;
     void *bufs[] = {p->d_t2, p->d_t1, p->d_colx, p->d_coly, p->d_velx, p->d_vely,
                     p->d_slab_cx, p->d_slab_cy, p->d_slab_x2thr, p->d_vel2, p->d_w, p->d_hconv,
-                    p->d_stage};
+                    p->d_stage, p->d_pix_v2, p->d_pix_r};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     ktm_tables_free(&p->tb);
@@ -1095,4 +1277,83 @@ extern "C" int ktm_copy_to_host(void *dst, const void *src, size_t bytes)
 extern "C" int ktm_device_sync(void)
 {
     return hipDeviceSynchronize() == hipSuccess ? KTM_OK : KTM_ERR_DEVICE;
+}
+
+static int pixel_tables(const ktm_plan *cp, hipStream_t hs)
+{
+    ktm_plan *p = const_cast<ktm_plan *>(cp);
+    if (p->d_pix_v2) return KTM_OK;
+    const ktm_setup *s = &p->tb.s;
+    const size_t n = (size_t)s->nxp * s->nyp * s->ns;
+    if (hipMalloc((void **)&p->d_pix_v2, n * sizeof(float)) != hipSuccess) return KTM_ERR_NOMEM;
+    if (hipMalloc((void **)&p->d_pix_r, n * sizeof(float)) != hipSuccess) {
+        (void)hipFree(p->d_pix_v2);
+        p->d_pix_v2 = nullptr;
+        return KTM_ERR_NOMEM;
+    }
+    hipLaunchKernelGGL(k_pixel_velocity, dim3(4096), dim3(256), 0, hs, p->d_vel2, p->d_velx,
+                       p->d_vely, p->d_t1, s->nxp, s->nyp, s->ns, p->cfg.nvel_xlines, s->nsm,
+                       p->d_pix_v2, p->d_pix_r);
+    HIP_TRY(hipGetLastError());
+    return KTM_OK;
+}
+
+extern "C" int ktm_demigrate(const ktm_plan *p, const float *km, const ktm_trace_geom *geom,
+                             int64_t ntr, float *traces, void *stream)
+{
+    if (!p || ntr < 0) return KTM_ERR_ARG;
+    if (!ntr) return KTM_OK;
+    if (!km || !geom || !traces) return KTM_ERR_ARG;
+    hipStream_t hs = (hipStream_t)stream;
+    int st = pixel_tables(p, hs);
+    if (st != KTM_OK) return st;
+    const ktm_setup *s = &p->tb.s;
+    DemigArgs a;
+    memset(&a, 0, sizeof a);
+    a.km = km; a.geom = geom; a.ntr = ntr; a.traces = traces;
+    a.pv2 = p->d_pix_v2; a.pr = p->d_pix_r; a.t2 = p->d_t2; a.w = p->d_w;
+    a.colx = p->d_colx; a.coly = p->d_coly;
+    a.slab_cx = p->d_slab_cx; a.slab_cy = p->d_slab_cy; a.slab_x2thr = p->d_slab_x2thr;
+    a.ap_x2thr = p->tb.ap_x2thr; a.aperture = (float)p->cfg.aperture;
+    a.dt = s->dt; a.rdt = 1.0f / s->dt;
+    a.ns = s->ns; a.nh = s->nh; a.nxp = s->nxp; a.nyp = s->nyp; a.nx = s->nx; a.ny = s->ny;
+    a.jobs_y = p->cfg.jobs_y;
+    typedef void (*fn_t)(const DemigArgs);
+    fn_t fn;
+    if (s->ns <= 512) fn = k_demigrate<8, 1>;
+    else if (s->ns <= 1024) fn = k_demigrate<8, 2>;
+    else if (s->ns <= 1536) fn = k_demigrate<12, 2>;
+    else if (s->ns <= 2048) fn = k_demigrate<8, 4>;
+    else fn = k_demigrate<12, 4>;
+    const size_t lds = sizeof(float) * (size_t)(s->ns + 4);
+    for (int64_t i0 = 0; i0 < ntr; i0 += 1 << 30) {
+        const int64_t n = ntr - i0 < (1 << 30) ? ntr - i0 : (1 << 30);
+        DemigArgs b = a;
+        b.geom = geom + i0;
+        b.traces = traces + i0 * s->ns;
+        b.ntr = n;
+        hipLaunchKernelGGL(fn, dim3((unsigned)n), dim3(8 * KTM_WAVE), lds, hs, b);
+    }
+    HIP_TRY(hipGetLastError());
+    return KTM_OK;
+}
+
+extern "C" int ktm_poststack_adjoint(const ktm_plan *p, const float *img, const float *fold,
+                                     float *km_bar, void *stream)
+{
+    if (!p || !img || !fold || !km_bar) return KTM_ERR_ARG;
+    const ktm_setup *s = &p->tb.s;
+    const int slots = (s->ns + PS_NT - 1) / PS_NT;
+    if (slots > 12) return KTM_ERR_UNSUPPORTED;
+    typedef void (*ps_fn)(const float *, const float *, int64_t, int, float, const double *, float *);
+    ps_fn fn = slots <= 4 ? k_poststack_adjoint<4> : slots <= 8 ? k_poststack_adjoint<8>
+                                                                 : k_poststack_adjoint<12>;
+    const int64_t ntraces = (int64_t)s->nxp * s->nyp * s->nh;
+    const size_t lds = sizeof(double) * (size_t)(2 * s->ns + s->ns * PS_TR + (s->ns + 1) * PS_TR);
+    const int64_t nblk = (ntraces + PS_TR - 1) / PS_TR;
+    HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(fn, dim3((unsigned)nblk), dim3(PS_NT), lds, (hipStream_t)stream, img, fold,
+                       ntraces, s->ns, s->dt, p->d_hconv, km_bar);
+    HIP_TRY(hipGetLastError());
+    return KTM_OK;
 }

@@ -173,6 +173,34 @@ class Plan:
               "ktm_poststack")
         return out
 
+    # -- adjoint pair (CLOSED prep) ------------------------------------------
+    def migrate_lin(self, traces, coords, *, kahan: bool = False):
+        """km = L^T d: the linear migration operator (CLOSED filter) on a zeroed
+        image; returns (km, fold)."""
+        from ._lib import PREP_CLOSED
+        prepared, geom = self.prep(traces, coords, PREP_CLOSED, TRACE_F32)
+        km, fold, comp = self.new_image(kahan=kahan, device=traces.device)
+        self.migrate_block(prepared, geom, km, fold, comp, mode=PREP_CLOSED,
+                           accum=ACCUM_KAHAN if kahan else ACCUM_PLAIN)
+        return km, fold
+
+    def demigrate(self, km, geom):
+        """d = L km: the exact transpose of ``migrate_lin`` (modelling)."""
+        import torch
+        ntr = geom.shape[0]
+        out = torch.empty((ntr, self.ns), dtype=torch.float32, device=km.device)
+        check(lib().ktm_demigrate(self._h, _ptr(km.contiguous()), _ptr(geom), ntr, _ptr(out),
+                                  _stream()), "ktm_demigrate")
+        return out
+
+    def poststack_adjoint(self, image, fold):
+        """km = S^T image, the transpose of ``poststack`` for a fixed fold."""
+        import torch
+        out = torch.empty_like(image)
+        check(lib().ktm_poststack_adjoint(self._h, _ptr(image.contiguous()), _ptr(fold), _ptr(out),
+                                          _stream()), "ktm_poststack_adjoint")
+        return out
+
     def migrate(self, traces, coords, *, block: int | None = None, mode: int = PREP_REFERENCE,
                 dtype: int = TRACE_F32, accum: int = ACCUM_PLAIN, job: int = -1):
         """All traces in blocks of ``ntblock``: returns (km, fold, contributions)."""

@@ -1,0 +1,101 @@
+"""Adjoint-pair wrappers: autograd, dot-product test and least-squares
+migration (CGLS) on the GPU kernels.
+
+L^T (migration, ``Plan.migrate_lin``) maps traces to the raw image, L
+(demigration, ``Plan.demigrate``) maps an image to traces; with
+``poststack=True`` the chain includes the derivative / fold / low-pass of
+Kirchhoff.c:1457-1479 (``Plan.poststack`` and its transpose).  Scalars of
+the iterations (dot products, step lengths, residual norms) are FP64.
+"""
+from __future__ import annotations
+
+import torch
+
+from .ktm import Plan
+
+
+def dot64(a: torch.Tensor, b: torch.Tensor) -> float:
+    """<a, b> accumulated in FP64."""
+    return float(torch.dot(a.reshape(-1).double(), b.reshape(-1).double()).item())
+
+
+class KirchhoffMigration(torch.autograd.Function):
+    """image = L^T traces (optionally followed by the post-stack chain);
+    backward applies L (and the post-stack transpose)."""
+
+    @staticmethod
+    def forward(ctx, traces, plan: Plan, coords, poststack: bool = False):
+        km, fold = plan.migrate_lin(traces.contiguous(), coords, kahan=True)
+        geom = plan.geometry(coords)
+        ctx.plan, ctx.geom, ctx.fold, ctx.poststack = plan, geom, fold, poststack
+        if poststack:
+            return plan.poststack(km, fold)
+        return km
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        g = grad_out.contiguous()
+        if ctx.poststack:
+            g = ctx.plan.poststack_adjoint(g, ctx.fold)
+        return ctx.plan.demigrate(g, ctx.geom), None, None, None
+
+
+def migrate_op(traces, plan: Plan, coords, poststack: bool = False):
+    return KirchhoffMigration.apply(traces, plan, coords, poststack)
+
+
+class Operator:
+    """L / L^T on one survey with the geometry cached.  L^T accumulates with
+    the Kahan-compensated FP32 kernel by default: at ~2e4 traces per image
+    sample a plain FP32 sum carries ~1e-5 relative error, the size of the
+    adjointness tolerance."""
+
+    def __init__(self, plan: Plan, coords: torch.Tensor, kahan: bool = True):
+        self.plan = plan
+        self.coords = coords
+        self.kahan = kahan
+        self.geom = plan.geometry(coords)
+
+    def adjoint(self, d):      # L^T: traces -> image
+        return self.plan.migrate_lin(d, self.coords, kahan=self.kahan)[0]
+
+    def forward(self, m):      # L: image -> traces
+        return self.plan.demigrate(m, self.geom)
+
+
+def dot_test(op: Operator, d: torch.Tensor, m: torch.Tensor) -> tuple[float, float, float]:
+    """(<L^T d, m>, <d, L m>, relative difference)."""
+    lhs = dot64(op.adjoint(d), m)
+    rhs = dot64(d, op.forward(m))
+    return lhs, rhs, abs(lhs - rhs) / max(abs(lhs), abs(rhs), 1e-300)
+
+
+def cgls(op: Operator, d_obs: torch.Tensor, niter: int = 20, m0: torch.Tensor | None = None,
+         callback=None):
+    """Least-squares migration: min_m ||L m - d_obs||^2 by conjugate gradients
+    on the normal equations (CGLS), FP32 vectors, FP64 scalars.  Returns
+    (m, history) with history[k] = (||r_k||, ||L^T r_k||)."""
+    m = torch.zeros_like(op.adjoint(d_obs)) if m0 is None else m0.clone()
+    r = d_obs - op.forward(m) if m0 is not None else d_obs.clone()
+    s = op.adjoint(r)
+    p = s.clone()
+    gamma = dot64(s, s)
+    hist = [(dot64(r, r) ** 0.5, gamma ** 0.5)]
+    for k in range(niter):
+        q = op.forward(p)
+        qq = dot64(q, q)
+        if qq <= 0.0:
+            break
+        alpha = gamma / qq
+        m += alpha * p
+        r -= alpha * q
+        s = op.adjoint(r)
+        gamma_new = dot64(s, s)
+        hist.append((dot64(r, r) ** 0.5, gamma_new ** 0.5))
+        if callback:
+            callback(k, m, hist[-1])
+        if gamma_new == 0.0:
+            break
+        p = s + (gamma_new / gamma) * p
+        gamma = gamma_new
+    return m, hist

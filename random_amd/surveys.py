@@ -157,6 +157,25 @@ def traces_numpy(sv: Survey, noise: bool = True, chunk: int = 512) -> np.ndarray
     return out
 
 
+def line2d_lsm(seed=20240021) -> Survey:
+    """Configuration 2: the 2-D line at 200 shots x 240 receivers, 1501
+    samples, 1600 image columns; 50 point scatterers for the reflectivity."""
+    return line2d(nshots=200, nrecv_side=120, ns=1501, ncols=1600, nscat=50, seed=seed)
+
+
+def point_reflectivity(sv: Survey, image_shape) -> np.ndarray:
+    """Image-domain model with one unit-amplitude spike (times the scatterer
+    amplitude) at each scatterer's (column, t0 sample)."""
+    m = np.zeros(image_shape, np.float32)
+    for x, y, t0, a in sv.scatterers:
+        X = int(round(x / sv.cfg.iline_distance)) - sv.cfg.start_iline
+        Y = int(round(y / sv.cfg.xline_distance)) - sv.cfg.start_xline if image_shape[1] > 1 else 0
+        T = int(round(t0 / sv.dt))
+        if 0 <= X < image_shape[0] and 0 <= Y < image_shape[1] and 0 <= T < image_shape[3]:
+            m[X, Y, 0, T] += a
+    return m
+
+
 def subset(sv: Survey, idx) -> Survey:
     """The same survey restricted to traces ``idx`` (order kept)."""
     import dataclasses
