@@ -189,6 +189,40 @@ int ktm_demigrate(const ktm_plan *plan, const float *km, const ktm_trace_geom *g
 int ktm_poststack_adjoint(const ktm_plan *plan, const float *image, const float *fold,
                           float *km, void *stream);
 
+/* ---- velocity gradient of a gather-flatness objective (device) ----
+ * A differentiable variant of the migration loop (Kirchhoff.c:1332-1404)
+ * for velocity analysis; no counterpart in the reference, whose
+ * nearest-node velocity (1357-1358,1376) and floor(dsr/dt) (1384) have zero
+ * derivative almost everywhere.  Lines only (one crossline column):
+ *   v(t,X)     = sum_ij V[i][j] B(t/dtn + 1 - i) B(X/dxn + 1 - j), uniform cubic
+ *                B-spline on nt_nodes x nx_nodes, dtn = (ns-1)/(nt_nodes-3),
+ *                dxn = (nxp-1)/(nx_nodes-3);
+ *   q          = (sqrt(t2 + dIS2/v^2) + sqrt(t2 + dIR2/v^2)) / dt (not floored);
+ *   I[X][h][t] = sum over traces of class h in aperture of the CLOSED filtered
+ *                trace at q, Catmull-Rom cubic interpolation (C1 in q);
+ *   J          = sum_{X,t} sum_{h<nh-1} (I[X][h+1][t] - I[X][h][t])^2.
+ * dJ/dV comes from a hand-written backward pass that recomputes traveltimes
+ * per time chunk of `chunk` (<= 256) samples instead of storing them. */
+typedef struct ktm_vgrad ktm_vgrad;
+int ktm_vgrad_create(ktm_vgrad **out, const ktm_config *cfg, int ns, float dt, int nt_nodes,
+                     int nx_nodes, int chunk);
+int ktm_vgrad_destroy(ktm_vgrad *vg);
+/* Use a block of CLOSED-prepared traces (ktm_prep with KTM_PREP_CLOSED,
+ * F32; row = ktm_plan_trace_row) and their geometry, both on the device and
+ * kept alive by the caller.  Bins the traces by (offset class, CMP) on the
+ * host; synchronises the stream. */
+int ktm_vgrad_bind(ktm_vgrad *vg, const float *prepared, int64_t row,
+                   const ktm_trace_geom *geom, int64_t ntr, void *stream);
+/* v(t,X) of node velocities V[nt_nodes][nx_nodes] on the image grid
+ * [nxp][ns] (device); KTM_ERR_VELOCITY if some v is not positive and finite.
+ * Synchronises the stream. */
+int ktm_vgrad_velocity(ktm_vgrad *vg, const float *V, float *v, void *stream);
+/* J (host, FP64) for node velocities V (device); if dJdV != NULL also the
+ * gradient [nt_nodes][nx_nodes] (device, FP64); if cig != NULL the gathers
+ * I [nxp][nh][ns] (device).  Synchronises the stream. */
+int ktm_vgrad_eval(ktm_vgrad *vg, const float *V, double *J, double *dJdV, float *cig,
+                   void *stream);
+
 /* Host convenience: whole reference job loop for all jobs (Kirchhoff.c:
  * 347-350 on one rank): prep + migrate in blocks of cfg->ntblock + post-stack.
  * Host arrays; km/fold/image are [nxp][nyp][nh][ns] (any may be NULL). */

@@ -63,6 +63,9 @@ _PROTOS = {
                                      _i32, _i32, _i32, _i32, _vp]),
     "orc_segy_file_header": (None, [C.POINTER(OrcParams), C.POINTER(OrcGrid), _i32, _f32, _i32, _vp]),
     "orc_count_size": (_i32, [_vp, _i64, C.POINTER(_i32), C.POINTER(_f32), C.POINTER(_i32)]),
+    "orc_vgrad": (C.c_double, [_i32, C.c_double, C.c_double, _i32, _i32, _vp, _f32, _vp, _vp, _vp, _i64,
+                               _i32, _i32, _vp, _vp, _vp]),
+    "orc_vgrad_velocity": (None, [_i32, _i32, _i32, _i32, _vp, _vp]),
 }
 
 _libs: dict = {}
@@ -226,3 +229,47 @@ def count_size(head: bytes, length: int):
     if rc != 0:
         raise ValueError("file too short")
     return ns.value, dt.value, nt.value
+
+
+def column_coords(cfg, n: int, axis: int = 0) -> np.ndarray:
+    """Image column coordinates IX (axis 0) or IY (axis 1) as float32,
+    Kirchhoff.c:1346-1347 (origin dropped when image_origin_zero)."""
+    start = 0.0 if cfg.image_origin_zero else float(cfg.start_xline if axis else cfg.start_iline)
+    inc = cfg.inc_xline if axis else cfg.inc_iline
+    d = cfg.xline_distance if axis else cfg.iline_distance
+    return np.array([(start + c * inc) * d for c in range(n)], np.float64).astype(np.float32)
+
+
+def filtered_traces_fp64(traces: np.ndarray, dt: float, tm: int = 2) -> np.ndarray:
+    """F[ntr][ns] of every trace in FP64 (the double integral of
+    Kirchhoff.c:1312-1328 and the taps of 1392 with every float widened)."""
+    ns = traces.shape[1]
+    return np.stack([filtered_trace(trace_prep(tr, dt, tm, fp64=True), tm, ns) for tr in traces])
+
+
+def vgrad_velocity(ns: int, nxp: int, V: np.ndarray) -> np.ndarray:
+    V = np.ascontiguousarray(V, np.float64)
+    v = np.zeros((nxp, ns), np.float64)
+    lib().orc_vgrad_velocity(ns, nxp, V.shape[0], V.shape[1], _vpn(V), _vpn(v))
+    return v
+
+
+def vgrad(cfg, ns, dt, F, coords, V, cig=False, grad=True):
+    """(J, dJdV or None, cig or None) of the differentiable offset-class
+    migration (FP64).  F: [ntr][ns] filtered traces; V: [nt_nodes][nx_nodes]."""
+    g = grid(cfg)
+    nxp = g.nx * cfg.jobs_x
+    if g.ny * cfg.jobs_y != 1:
+        raise ValueError("vgrad: lines only")
+    geo, off = trace_geometry(cfg, coords)
+    colx = column_coords(cfg, nxp, 0)
+    coly = column_coords(cfg, 1, 1)[0]
+    F = np.ascontiguousarray(F, np.float64)
+    V = np.ascontiguousarray(V, np.float64)
+    c = np.zeros((nxp, cfg.nh, ns), np.float64) if cig else None
+    d = np.zeros(V.shape, np.float64) if grad else None
+    J = lib().orc_vgrad(ns, float(np.float32(dt)), float(cfg.aperture), cfg.nh, nxp, _vpn(colx),
+                        float(coly), _vpn(F), _vpn(geo), _vpn(off), F.shape[0], V.shape[0],
+                        V.shape[1], _vpn(V), None if c is None else _vpn(c),
+                        None if d is None else _vpn(d))
+    return J, d, c

@@ -634,3 +634,130 @@ int orc_count_size(const uint8_t *head, int64_t length, int *ns, float *dt, int 
     *nt = (int)((length - 3600) / (240 + ns1 * 4));
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Differentiable offset-class migration and its velocity gradient           */
+/* (an extension of the reference operator; no counterpart in Kirchhoff.c,   */
+/* whose nearest-neighbour velocity and floor(T) have zero derivative almost */
+/* everywhere).  FP64 throughout.                                            */
+/* ------------------------------------------------------------------------ */
+
+/* centred cubic B-spline, support (-2, 2) */
+static double bspline3(double z)
+{
+    const double a = fabs(z);
+    if (a >= 2.0) return 0.0;
+    if (a >= 1.0) return (2.0 - a) * (2.0 - a) * (2.0 - a) / 6.0;
+    return 2.0 / 3.0 - a * a + 0.5 * a * a * a;
+}
+
+void orc_vgrad_velocity(int ns, int nxp, int nt_nodes, int nx_nodes, const double *V,
+                        double *v)
+{
+    const double dtn = (double)(ns - 1) / (nt_nodes - 3);
+    const double dxn = nxp > 1 ? (double)(nxp - 1) / (nx_nodes - 3) : 1.0;
+    for (int X = 0; X < nxp; ++X)
+        for (int t = 0; t < ns; ++t) {
+            double s = 0.0;
+            for (int i = 0; i < nt_nodes; ++i) {
+                const double bt = bspline3(t / dtn + 1.0 - i);
+                if (bt == 0.0) continue;
+                for (int j = 0; j < nx_nodes; ++j) {
+                    const double bx = bspline3(X / dxn + 1.0 - j);
+                    if (bx != 0.0) s += V[(int64_t)i * nx_nodes + j] * bt * bx;
+                }
+            }
+            v[(int64_t)X * ns + t] = s;
+        }
+}
+
+/* tap k of the filtered trace, zero outside [0, ns) */
+static double ftap(const double *F, int ns, int64_t k)
+{
+    return (k >= 0 && k < ns) ? F[k] : 0.0;
+}
+
+double orc_vgrad(int ns, double dt, double aperture, int nh, int nxp, const float *colx,
+                 float coly, const double *F, const float *geo, const int32_t *off,
+                 int64_t ntr, int nt_nodes, int nx_nodes, const double *V, double *cig,
+                 double *dJdV)
+{
+    double *v = (double *)malloc(sizeof(double) * (size_t)nxp * ns);
+    double *I = (double *)calloc((size_t)nh * ns, sizeof(double));
+    double *S = (double *)calloc((size_t)nh * ns, sizeof(double));   /* sum of dp/dv */
+    double *gv = (double *)calloc((size_t)nxp * ns, sizeof(double)); /* dJ/dv(X,t) */
+    orc_vgrad_velocity(ns, nxp, nt_nodes, nx_nodes, V, v);
+    double J = 0.0;
+    for (int X = 0; X < nxp; ++X) {
+        memset(I, 0, sizeof(double) * (size_t)nh * ns);
+        memset(S, 0, sizeof(double) * (size_t)nh * ns);
+        const float IX = colx[X], IY = coly;
+        for (int64_t i = 0; i < ntr; ++i) {
+            const int h = off[i];
+            if (h < 0 || h >= nh) continue;
+            const float *g = geo + 6 * i;
+            /* aperture gate of Kirchhoff.c:1353-1354 (float distance) */
+            const float cx = IX - g[4], cy = IY - g[5];
+            const float dist = (float)sqrt((double)(cx * cx + cy * cy));
+            if (!(dist < aperture)) continue;
+            const double a = ((double)IX - g[0]) * ((double)IX - g[0]) +
+                             ((double)IY - g[1]) * ((double)IY - g[1]);
+            const double b = ((double)IX - g[2]) * ((double)IX - g[2]) +
+                             ((double)IY - g[3]) * ((double)IY - g[3]);
+            const double *Fi = F + i * (int64_t)ns;
+            for (int t = 0; t < ns; ++t) {
+                const double vv = v[(int64_t)X * ns + t];
+                const double th = 0.5 * t * dt, t2 = th * th;
+                const double s1 = sqrt(t2 + a / (vv * vv)), s2 = sqrt(t2 + b / (vv * vv));
+                double q = (s1 + s2) / dt;
+                if (q > ns + 2.0) q = ns + 2.0;
+                const double Tf = floor(q), u = q - Tf;
+                const int64_t T = (int64_t)Tf;
+                const double f0 = ftap(Fi, ns, T - 1), f1 = ftap(Fi, ns, T);
+                const double f2 = ftap(Fi, ns, T + 1), f3 = ftap(Fi, ns, T + 2);
+                /* Catmull-Rom cubic through f0..f3 at u in [0,1) */
+                const double c1 = 0.5 * (f2 - f0);
+                const double c2 = f0 - 2.5 * f1 + 2.0 * f2 - 0.5 * f3;
+                const double c3 = 1.5 * (f1 - f2) + 0.5 * (f3 - f0);
+                I[(int64_t)h * ns + t] += f1 + u * (c1 + u * (c2 + u * c3));
+                if (dJdV) {
+                    const double dp = c1 + u * (2.0 * c2 + 3.0 * u * c3);
+                    /* dq/dv = -(a/s1 + b/s2) / (v^3 dt); s = 0 only with a = 0 */
+                    const double ds = (s1 > 0.0 ? a / s1 : 0.0) + (s2 > 0.0 ? b / s2 : 0.0);
+                    S[(int64_t)h * ns + t] += dp * (-ds / (vv * vv * vv * dt));
+                }
+            }
+        }
+        for (int t = 0; t < ns; ++t) {
+            double gsum = 0.0;
+            for (int h = 0; h < nh; ++h) {
+                const double Ih = I[(int64_t)h * ns + t];
+                const double dn = h + 1 < nh ? I[(int64_t)(h + 1) * ns + t] - Ih : 0.0;
+                const double dp = h > 0 ? Ih - I[(int64_t)(h - 1) * ns + t] : 0.0;
+                if (h + 1 < nh) J += dn * dn;
+                gsum += 2.0 * (dp - dn) * S[(int64_t)h * ns + t];
+                if (cig) cig[((int64_t)X * nh + h) * ns + t] = Ih;
+            }
+            gv[(int64_t)X * ns + t] = gsum;
+        }
+    }
+    if (dJdV) {
+        const double dtn = (double)(ns - 1) / (nt_nodes - 3);
+        const double dxn = nxp > 1 ? (double)(nxp - 1) / (nx_nodes - 3) : 1.0;
+        for (int i = 0; i < nt_nodes; ++i)
+            for (int j = 0; j < nx_nodes; ++j) {
+                double s = 0.0;
+                for (int X = 0; X < nxp; ++X) {
+                    const double bx = bspline3(X / dxn + 1.0 - j);
+                    if (bx == 0.0) continue;
+                    for (int t = 0; t < ns; ++t) {
+                        const double bt = bspline3(t / dtn + 1.0 - i);
+                        if (bt != 0.0) s += gv[(int64_t)X * ns + t] * bt * bx;
+                    }
+                }
+                dJdV[(int64_t)i * nx_nodes + j] = s;
+            }
+    }
+    free(v); free(I); free(S); free(gv);
+    return J;
+}

@@ -26,6 +26,9 @@
  *   orc_poststack         direct DFT in exact rational/np.fft agreement,
  *                         linear-phase impulse response
  *   orc_segy_*            byte layout of READ_SEGY.c:29-115 (field order)
+ *   orc_vgrad             FP64 central differences of J at every node, spline
+ *                         partition of unity, Catmull-Rom reproduction of
+ *                         quadratics, zero-offset identity I = F
  */
 #ifndef KTM_ORACLE_H
 #define KTM_ORACLE_H
@@ -141,6 +144,29 @@ void  orc_segy_file_header(const orc_params *p, const orc_grid *g, int ns, float
 /* Kirchhoff.c:633-722 / READ_SEGY.c:169-209: ns, dt, nt of a file. */
 int   orc_count_size(const uint8_t *head3720, int64_t length, int *ns, float *dt,
                      int *nt);
+
+/* Differentiable offset-class migration (an extension with no counterpart in
+ * Kirchhoff.c, whose nearest-neighbour velocity 1357-1358,1376 and floor(T)
+ * 1384 have zero derivative almost everywhere).  FP64 throughout.
+ *   v(t,X) = sum_ij V[i][j] B((t/dtn) + 1 - i) B((X/dxn) + 1 - j), B the centred
+ *            cubic B-spline, dtn = (ns-1)/(nt_nodes-3), dxn = (nxp-1)/(nx_nodes-3)
+ *   q      = (sqrt(t2 + a/v^2) + sqrt(t2 + b/v^2)) / dt, t2 = (t dt / 2)^2,
+ *            a, b the squared source/receiver distances (Kirchhoff.c:1348-1351)
+ *   I[X][h][t] = sum over traces of class h passing the aperture gate of
+ *            Kirchhoff.c:1354 of the Catmull-Rom interpolant of F at q
+ *            (F zero outside [0, ns), q clamped to ns + 2)
+ *   J      = sum_{X,t} sum_{h < nh-1} (I[X][h+1][t] - I[X][h][t])^2
+ * F: [ntr][ns] filtered traces; geo[ntr][6] and off[ntr] from
+ * orc_trace_geometry; colx[nxp] the column coordinates, coly the line's.
+ * Fills cig[nxp][nh][ns] and dJdV[nt_nodes][nx_nodes] when non-NULL (the
+ * analytic gradient); returns J. */
+double orc_vgrad(int ns, double dt, double aperture, int nh, int nxp, const float *colx,
+                 float coly, const double *F, const float *geo, const int32_t *off,
+                 int64_t ntr, int nt_nodes, int nx_nodes, const double *V, double *cig,
+                 double *dJdV);
+/* v(t, X) of the spline above, [nxp][ns]. */
+void  orc_vgrad_velocity(int ns, int nxp, int nt_nodes, int nx_nodes, const double *V,
+                         double *v);
 
 #ifdef __cplusplus
 }

@@ -112,6 +112,11 @@ _PROTOS = {
     "ktm_gate_threshold": (C.c_float, [C.c_double]),
     "ktm_demigrate": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp]),
     "ktm_poststack_adjoint": (_i32, [_vp, _vp, _vp, _vp, _vp]),
+    "ktm_vgrad_create": (_i32, [C.POINTER(_vp), C.POINTER(KtmConfig), _i32, _f32, _i32, _i32, _i32]),
+    "ktm_vgrad_destroy": (_i32, [_vp]),
+    "ktm_vgrad_bind": (_i32, [_vp, _vp, _i64, _vp, _i64, _vp]),
+    "ktm_vgrad_velocity": (_i32, [_vp, _vp, _vp, _vp]),
+    "ktm_vgrad_eval": (_i32, [_vp, _vp, C.POINTER(C.c_double), _vp, _vp, _vp]),
     "ktm_selftest_traveltime": (_i32, [_vp, _i64, _f32, _i32, _vp, _vp, _vp, _vp, _vp]),
 }
 

@@ -63,7 +63,7 @@ def _velocity_grid(cfg: Config, ns: int, dt: float, v0: float, vt: float, vx: fl
 
 def line2d(nshots=40, nrecv_side=60, shot_dx=50.0, recv_dx=12.5, ns=1001, ncols=800,
            col_dx=6.25, aperture=2000.0, nscat=20, seed=20240001, jobs_x=1,
-           nh=1, oh=0.0, dh=2000.0, x0=1522.0) -> Survey:
+           nh=1, oh=0.0, dh=2000.0, x0=1522.0, t0_range=(0.3, 3.5)) -> Survey:
     """2-D split-spread line (configuration 0 shape by default): shots every
     ``shot_dx`` m, ``nrecv_side`` receivers each side at ``recv_dx`` m,
     ``ncols`` image columns at ``col_dx`` m starting at 0, v = 1500 + 400 t.
@@ -87,7 +87,7 @@ def line2d(nshots=40, nrecv_side=60, shot_dx=50.0, recv_dx=12.5, ns=1001, ncols=
     scat = np.zeros((nscat, 4), np.float32)
     scat[:, 0] = rng.uniform(cmp.min(), cmp.max(), nscat)
     scat[:, 1] = 0.0
-    scat[:, 2] = rng.uniform(0.3, 3.5, nscat)
+    scat[:, 2] = rng.uniform(t0_range[0], t0_range[1], nscat)
     scat[:, 3] = rng.uniform(0.5, 1.0, nscat)
     vel = _velocity_grid(cfg, ns, DT, 1500.0, 400.0, 0.0)
     return Survey("line2d", cfg, ns, float(DT), coords, scat, vel, 1500.0, 400.0, 0.0, 10.0, seed)
@@ -161,6 +161,17 @@ def line2d_lsm(seed=20240021) -> Survey:
     """Configuration 2: the 2-D line at 200 shots x 240 receivers, 1501
     samples, 1600 image columns; 50 point scatterers for the reflectivity."""
     return line2d(nshots=200, nrecv_side=120, ns=1501, ncols=1600, nscat=50, seed=seed)
+
+
+def line2d_vgrad(seed=20240041, **kw) -> Survey:
+    """Configuration 4: 600 shots x 240 split-spread receivers (144,000
+    traces), 2001 samples, 2000 image columns at 16 m, 64 offset classes of
+    23.5 m (class = floor((offset - 6.25) / 23.5), offsets 12.5..1500 m),
+    aperture 2000 m, 50 scatterers."""
+    args = dict(nshots=600, nrecv_side=120, ns=2001, ncols=2000, col_dx=16.0, aperture=2000.0,
+                nscat=50, nh=64, oh=6.25, dh=23.5, x0=1500.0, seed=seed)
+    args.update(kw)
+    return line2d(**args)
 
 
 def point_reflectivity(sv: Survey, image_shape) -> np.ndarray:
