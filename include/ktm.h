@@ -139,6 +139,26 @@ int ktm_prep(const ktm_plan *plan, const float *traces, const int32_t *coords,
              int64_t ntr, int prep_mode, int trace_dtype, void *prepared,
              ktm_trace_geom *geom, void *stream);
 
+/* ---- SEG-Y ingest on the device ----
+ * raw: ntr trace records exactly as stored in the file after its 3600-byte
+ * header (Kirchhoff.c:1281-1283: 240-byte header + ns big-endian sample
+ * words), on the device, 4-byte aligned.  format: the SEG-Y sample format
+ * code, 1 (IBM hex float, decoded as Kirchhoff.c:1284-1290) or 5 (IEEE). */
+enum { KTM_SEGY_IBM = 1, KTM_SEGY_IEEE = 5 };
+/* The block's prep in one pass per kernel: byte swap, IBM decode and header
+ * words 18-21 (Kirchhoff.c:1297,1302) inside the geometry and prep kernels;
+ * results identical to ktm_segy_read + ktm_prep.  coords ([ntr][4], may be
+ * NULL) receives the header words.  Replaces the per-trace fread/decode
+ * loop of Kirchhoff.c:1276-1328. */
+int ktm_ingest(const ktm_plan *plan, const void *raw, int64_t ntr, int format,
+               int prep_mode, int trace_dtype, void *prepared, ktm_trace_geom *geom,
+               int32_t *coords, void *stream);
+/* Decode raw records into FP32 traces [ntr][ns] and header words 18-21
+ * [ntr][4] (either may be NULL), bit for bit as ktm_segy_read does on the
+ * host (NaN payloads aside). */
+int ktm_segy_decode(const void *raw, int64_t ntr, int ns, int format, float *traces,
+                    int32_t *coords, void *stream);
+
 /* km = 0, fold = 1, comp = 0 (Kirchhoff.c:1262-1274); comp may be NULL. */
 int ktm_image_reset(const ktm_plan *plan, float *km, float *fold, float *comp,
                     void *stream);

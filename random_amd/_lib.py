@@ -29,6 +29,7 @@ STATUS_NAMES = {
 PREP_REFERENCE, PREP_CLOSED = 0, 1
 TRACE_F32, TRACE_BF16, TRACE_F16 = 0, 1, 2
 ACCUM_PLAIN, ACCUM_KAHAN = 0, 1
+SEGY_IBM, SEGY_IEEE = 1, 5          # SEG-Y sample format codes
 
 
 class KtmError(RuntimeError):
@@ -112,6 +113,8 @@ _PROTOS = {
     "ktm_gate_threshold": (C.c_float, [C.c_double]),
     "ktm_demigrate": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp]),
     "ktm_poststack_adjoint": (_i32, [_vp, _vp, _vp, _vp, _vp]),
+    "ktm_ingest": (_i32, [_vp, _vp, _i64, _i32, _i32, _i32, _vp, _vp, _vp, _vp]),
+    "ktm_segy_decode": (_i32, [_vp, _i64, _i32, _i32, _vp, _vp, _vp]),
     "ktm_vgrad_create": (_i32, [C.POINTER(_vp), C.POINTER(KtmConfig), _i32, _f32, _i32, _i32, _i32]),
     "ktm_vgrad_destroy": (_i32, [_vp]),
     "ktm_vgrad_bind": (_i32, [_vp, _vp, _i64, _vp, _i64, _vp]),

@@ -4,6 +4,10 @@
  *
  * Kernels (every one replaces a loop of Kirchhoff.c's kirchhoff()):
  *   k_geometry      per-trace rotation / CMP / offset class   Kirchhoff.c:1296-1310
+ *                   (k_geometry_segy: from the raw trace headers)
+ *   k_segy_decode   raw SEG-Y records -> FP32 traces + header words,
+ *                   Kirchhoff.c:1281-1290 (the prep kernels below decode
+ *                   records themselves when ingesting a raw block)
  *   k_prep_ref      FP32 double integral + trange=2 taps       Kirchhoff.c:1312-1328,1391-1392
  *   k_prep_closed   same operator as a 3-tap filter (FP32), from FP32 or
  *                   BF16/FP16-stored traces              (exact rewrite of 1312-1392)
@@ -107,9 +111,68 @@ __global__ void k_geometry(GeoArgs g, const int32_t *__restrict__ coords, int64_
     geom[i] = trace_geometry(g, coords + 4 * i);
 }
 
+/* The same from raw SEG-Y trace records: header words 18-21 (sx, sy, gx,
+ * gy; bytes 73-88, big-endian, Kirchhoff.c:1297,1302) of each record of
+ * rec_words 4-byte words; the header words also go to coords if given. */
+__global__ void k_geometry_segy(GeoArgs g, const uint32_t *__restrict__ raw, int64_t rec_words,
+                                int64_t ntr, ktm_trace_geom *__restrict__ geom,
+                                int32_t *__restrict__ coords)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntr) return;
+    int32_t c[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) c[k] = (int32_t)__builtin_bswap32(raw[i * rec_words + 18 + k]);
+    if (coords) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) coords[4 * i + k] = c[k];
+    }
+    geom[i] = trace_geometry(g, c);
+}
+
 /* ------------------------------------------------------------------------ */
 /* trace preparation                                                         */
 /* ------------------------------------------------------------------------ */
+
+/* Where the prep kernels read trace samples: FP32 traces [ntr][ns], or raw
+ * SEG-Y trace records as stored in the file (240-byte header + ns big-endian
+ * words, Kirchhoff.c:1281-1290) with IBM (format 1) or IEEE (format 5)
+ * samples.  Decoding inside the prep kernels makes the ingest of a block
+ * one pass: the byte swap and IBM decode of Kirchhoff.c:1284-1290 happen
+ * where the sample is consumed. */
+enum { SRC_F32 = 0, SRC_SEGY_IBM = 1, SRC_SEGY_IEEE = 2, SRC_Q_BF16 = 3, SRC_Q_F16 = 4 };
+
+struct TraceSrc {
+    const void *p;
+    int64_t rec_words;      /* 4-byte words per trace: ns, or 60 + ns for a record */
+};
+
+/* IBM hex float -> float as Kirchhoff.c:1287-1289 evaluates it:
+ * (sign * fraction / 2^24) * (float)16^(exponent - 64), the power of 16
+ * rounded to float on its own (inf above 2^127, 0 below 2^-149; a zero
+ * fraction times inf is NaN).  The power is built from its bits: e4 is a
+ * multiple of 4, so no value falls between two floats. */
+__device__ __forceinline__ float ibm_word(uint32_t w)
+{
+    const float frac = (float)(w & 0x00ffffffu) * (1.0f / 16777216.0f);
+    const int e4 = 4 * (int)((w >> 24) & 0x7fu) - 256;
+    uint32_t pb;
+    if (e4 > 127) pb = 0x7f800000u;
+    else if (e4 >= -126) pb = (uint32_t)(e4 + 127) << 23;
+    else if (e4 >= -149) pb = 1u << (e4 + 149);
+    else pb = 0u;
+    const float sf = (w >> 31) ? -frac : frac;
+    return sf * __uint_as_float(pb);
+}
+
+template <int SRC>
+__device__ __forceinline__ float load_sample(const TraceSrc &s, int64_t i, int t)
+{
+    if (SRC == SRC_F32) return reinterpret_cast<const float *>(s.p)[i * s.rec_words + t];
+    const uint32_t w =
+        __builtin_bswap32(reinterpret_cast<const uint32_t *>(s.p)[i * s.rec_words + 60 + t]);
+    return SRC == SRC_SEGY_IBM ? ibm_word(w) : __uint_as_float(w);
+}
 
 /* REFERENCE mode: one lane per trace runs the two FP32 running sums of
  * Kirchhoff.c:1314-1324 in a single pass, keeping the last five values of
@@ -117,20 +180,20 @@ __global__ void k_geometry(GeoArgs g, const int32_t *__restrict__ coords, int64_
  *   F[T] = (float)(-2.0*p[T] + p[T-2] + p[T+2])         (Kirchhoff.c:1392)
  * as soon as p[T+2] exists.  p[-1] = p[1] and p[-2] = p[2] are the mirror pad
  * of Kirchhoff.c:1327-1328.  No scratch buffer is needed. */
-__global__ void k_prep_ref(const float *__restrict__ traces, int64_t ntr, int ns, int64_t row,
+template <int SRC>
+__global__ void k_prep_ref(const TraceSrc src, int64_t ntr, int ns, int64_t row,
                            float dt, const float *__restrict__ t2, float *__restrict__ out)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntr) return;
-    const float *tr = traces + i * (int64_t)ns;
     float *F = out + i * row;
-    const float last = tr[ns - 1];
-    float q = tr[0];        /* integral 1 at t */
+    const float last = load_sample<SRC>(src, i, ns - 1);
+    float q = load_sample<SRC>(src, i, 0);   /* integral 1 at t */
     float p = q;            /* integral 2 at t */
     float pm1 = 0.0f, pm2 = 0.0f, pm3 = 0.0f, pm4 = 0.0f;
     for (int t = 1; t <= ns + 1; ++t) {
         if (t < ns)
-            q = (float)((double)q + (double)tr[t] / (4.0 * (double)t2[t]));
+            q = (float)((double)q + (double)load_sample<SRC>(src, i, t) / (4.0 * (double)t2[t]));
         else
             q = q + last / dt / dt / (float)t / (float)t;
         pm4 = pm3; pm3 = pm2; pm2 = pm1; pm1 = p;
@@ -170,14 +233,16 @@ __device__ __forceinline__ float closed_tap(int T, float x0, float x1, float x2,
     return (4.0f * x0 + 4.0f * x1) + 2.0f * x2;
 }
 
-/* src: F32 raw traces [ntr][ns] (SRC = KTM_TRACE_F32) or BF16/FP16 traces
- * stored by k_quantize [ntr][row] with the per-trace scale in geom; the
- * stored value times the scale stands for the trace sample. */
+/* src: any 32-bit trace source of load_sample, or BF16/FP16 traces stored
+ * by k_quantize (SRC_Q_*, [ntr][row] halves, rec_words = row) with the
+ * per-trace scale in geom; the stored value times the scale stands for the
+ * trace sample. */
 template <int SRC>
-__global__ void k_prep_closed(const void *__restrict__ src, const ktm_trace_geom *__restrict__ geom,
+__global__ void k_prep_closed(const TraceSrc src, const ktm_trace_geom *__restrict__ geom,
                               int64_t ntr, int ns, int64_t row, const float *__restrict__ w,
                               float *__restrict__ out)
 {
+    constexpr bool Q16 = SRC == SRC_Q_BF16 || SRC == SRC_Q_F16;
     const int64_t i = blockIdx.y;
     const int T = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntr || T >= row) return;
@@ -185,17 +250,17 @@ __global__ void k_prep_closed(const void *__restrict__ src, const ktm_trace_geom
     if (T < ns) {
         float x[4];
         const int b = T >= 2 ? T : 0;
-        const float sc = SRC == KTM_TRACE_F32 ? 1.0f : geom[i].scale;
+        const float sc = Q16 ? geom[i].scale : 1.0f;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int t = b + k;
             const int tt = t < ns ? t : ns - 1;
             float tr;
-            if (SRC == KTM_TRACE_F32) {
-                tr = reinterpret_cast<const float *>(src)[i * (int64_t)ns + tt];
+            if (Q16) {
+                const uint16_t hq = reinterpret_cast<const uint16_t *>(src.p)[i * src.rec_words + tt];
+                tr = sc * (SRC == SRC_Q_BF16 ? bf16_to_f32_(hq) : f16_to_f32_(hq));
             } else {
-                const uint16_t hq = reinterpret_cast<const uint16_t *>(src)[i * row + tt];
-                tr = sc * (SRC == KTM_TRACE_BF16 ? bf16_to_f32_(hq) : f16_to_f32_(hq));
+                tr = load_sample<SRC>(src, i, tt);
             }
             x[k] = t <= ns + 1 ? tr * w[t] : 0.0f;
         }
@@ -232,21 +297,20 @@ __device__ __forceinline__ float f16_to_f32(uint16_t u)
  * feedback (the rounding error of sample t is carried into sample t+1), which
  * moves the quantisation noise towards the Nyquist frequency, where the
  * [1,2,1] anti-alias taps of the migration have their zero. */
-template <int DT>
-__global__ void k_quantize(const float *__restrict__ traces, int64_t ntr, int ns, int64_t row,
+template <int DT, int SRC>
+__global__ void k_quantize(const TraceSrc src, int64_t ntr, int ns, int64_t row,
                            uint16_t *__restrict__ q, ktm_trace_geom *__restrict__ geom)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntr) return;
-    const float *tr = traces + i * (int64_t)ns;
     float m = 0.0f;
-    for (int t = 0; t < ns; ++t) m = fmaxf(m, fabsf(tr[t]));
+    for (int t = 0; t < ns; ++t) m = fmaxf(m, fabsf(load_sample<SRC>(src, i, t)));
     const float scale = m > 0.0f ? m : 1.0f;
     const float inv = 1.0f / scale;
     uint16_t *o = q + i * row;
     float e = 0.0f;
     for (int t = 0; t < ns; ++t) {
-        const float v = tr[t] * inv + e;
+        const float v = load_sample<SRC>(src, i, t) * inv + e;
         const uint16_t h = DT == KTM_TRACE_BF16 ? f32_to_bf16(v) : f32_to_f16(v);
         o[t] = h;
         e = v - (DT == KTM_TRACE_BF16 ? bf16_to_f32(h) : f16_to_f32(h));
@@ -1061,6 +1125,56 @@ extern "C" int ktm_geometry(const ktm_plan *p, const int32_t *coords, int64_t nt
     return KTM_OK;
 }
 
+/* prep kernels of one trace source (geometry already written) */
+template <int SRC>
+static int prep_launch(const ktm_plan *p, const TraceSrc src, int64_t ntr, int prep_mode,
+                       int dtype, void *prepared, ktm_trace_geom *geom, hipStream_t s)
+{
+    const int ns = p->tb.s.ns;
+    if (dtype == KTM_TRACE_F32) {
+        if (prep_mode == KTM_PREP_REFERENCE) {
+            const int nt = 64;
+            hipLaunchKernelGGL(k_prep_ref<SRC>, dim3((unsigned)((ntr + nt - 1) / nt)), dim3(nt), 0,
+                               s, src, ntr, ns, p->row, p->tb.s.dt, p->d_t2, (float *)prepared);
+        } else {
+            if (ntr > 65535 * 1024LL) return KTM_ERR_UNSUPPORTED;
+            const int nt = 256;
+            for (int64_t i0 = 0; i0 < ntr; i0 += 65535) {
+                const int64_t n = ntr - i0 < 65535 ? ntr - i0 : 65535;
+                const TraceSrc si = {reinterpret_cast<const uint32_t *>(src.p) + i0 * src.rec_words,
+                                     src.rec_words};
+                hipLaunchKernelGGL(k_prep_closed<SRC>,
+                                   dim3((unsigned)((p->row + nt - 1) / nt), (unsigned)n), dim3(nt),
+                                   0, s, si, geom + i0, n, ns, p->row, p->d_w,
+                                   (float *)prepared + i0 * p->row);
+            }
+        }
+    } else {
+        const int nt = 64;
+        const dim3 grid((unsigned)((ntr + nt - 1) / nt));
+        if (dtype == KTM_TRACE_BF16)
+            hipLaunchKernelGGL((k_quantize<KTM_TRACE_BF16, SRC>), grid, dim3(nt), 0, s, src, ntr, ns,
+                               p->row, (uint16_t *)prepared, geom);
+        else
+            hipLaunchKernelGGL((k_quantize<KTM_TRACE_F16, SRC>), grid, dim3(nt), 0, s, src, ntr, ns,
+                               p->row, (uint16_t *)prepared, geom);
+    }
+    HIP_TRY(hipGetLastError());
+    return KTM_OK;
+}
+
+static int prep_check(const ktm_plan *p, int prep_mode, int dtype)
+{
+    if (prep_mode != KTM_PREP_REFERENCE && prep_mode != KTM_PREP_CLOSED) return KTM_ERR_ARG;
+    if (dtype != KTM_TRACE_F32 && dtype != KTM_TRACE_BF16 && dtype != KTM_TRACE_F16)
+        return KTM_ERR_ARG;
+    /* reduced-precision storage keeps the raw trace; its filter (the closed
+     * form) is applied when a block is migrated */
+    if (dtype != KTM_TRACE_F32 && prep_mode != KTM_PREP_CLOSED) return KTM_ERR_UNSUPPORTED;
+    (void)p;
+    return KTM_OK;
+}
+
 extern "C" int ktm_prep(const ktm_plan *p, const float *traces, const int32_t *coords,
                         int64_t ntr, int prep_mode, int dtype, void *prepared,
                         ktm_trace_geom *geom, void *stream)
@@ -1068,42 +1182,69 @@ extern "C" int ktm_prep(const ktm_plan *p, const float *traces, const int32_t *c
     if (!p || ntr < 0) return KTM_ERR_ARG;
     if (!ntr) return KTM_OK;
     if (!traces || !coords || !prepared || !geom) return KTM_ERR_ARG;
-    if (prep_mode != KTM_PREP_REFERENCE && prep_mode != KTM_PREP_CLOSED) return KTM_ERR_ARG;
-    if (dtype != KTM_TRACE_F32 && dtype != KTM_TRACE_BF16 && dtype != KTM_TRACE_F16)
-        return KTM_ERR_ARG;
-    /* reduced-precision storage keeps the raw trace; its filter (the closed
-     * form) is applied when a block is migrated */
-    if (dtype != KTM_TRACE_F32 && prep_mode != KTM_PREP_CLOSED) return KTM_ERR_UNSUPPORTED;
-    hipStream_t s = (hipStream_t)stream;
-    int st = ktm_geometry(p, coords, ntr, geom, stream);
+    int st = prep_check(p, prep_mode, dtype);
     if (st != KTM_OK) return st;
-    const int ns = p->tb.s.ns;
-    if (dtype == KTM_TRACE_F32) {
-        if (prep_mode == KTM_PREP_REFERENCE) {
-            const int nt = 64;
-            hipLaunchKernelGGL(k_prep_ref, dim3((unsigned)((ntr + nt - 1) / nt)), dim3(nt), 0, s,
-                               traces, ntr, ns, p->row, p->tb.s.dt, p->d_t2, (float *)prepared);
-        } else {
-            if (ntr > 65535 * 1024LL) return KTM_ERR_UNSUPPORTED;
-            const int nt = 256;
-            for (int64_t i0 = 0; i0 < ntr; i0 += 65535) {
-                const int64_t n = ntr - i0 < 65535 ? ntr - i0 : 65535;
-                hipLaunchKernelGGL(k_prep_closed<KTM_TRACE_F32>,
-                                   dim3((unsigned)((p->row + nt - 1) / nt), (unsigned)n), dim3(nt),
-                                   0, s, (const void *)(traces + i0 * ns), geom + i0, n, ns,
-                                   p->row, p->d_w, (float *)prepared + i0 * p->row);
-            }
-        }
-    } else {
-        const int nt = 64;
-        const dim3 grid((unsigned)((ntr + nt - 1) / nt));
-        if (dtype == KTM_TRACE_BF16)
-            hipLaunchKernelGGL(k_quantize<KTM_TRACE_BF16>, grid, dim3(nt), 0, s, traces, ntr, ns,
-                               p->row, (uint16_t *)prepared, geom);
-        else
-            hipLaunchKernelGGL(k_quantize<KTM_TRACE_F16>, grid, dim3(nt), 0, s, traces, ntr, ns,
-                               p->row, (uint16_t *)prepared, geom);
-    }
+    st = ktm_geometry(p, coords, ntr, geom, stream);
+    if (st != KTM_OK) return st;
+    const TraceSrc src = {traces, p->tb.s.ns};
+    return prep_launch<SRC_F32>(p, src, ntr, prep_mode, dtype, prepared, geom, (hipStream_t)stream);
+}
+
+extern "C" int ktm_ingest(const ktm_plan *p, const void *raw, int64_t ntr, int format,
+                          int prep_mode, int dtype, void *prepared, ktm_trace_geom *geom,
+                          int32_t *coords, void *stream)
+{
+    if (!p || ntr < 0) return KTM_ERR_ARG;
+    if (!ntr) return KTM_OK;
+    if (!raw || !prepared || !geom || ((uintptr_t)raw & 3u)) return KTM_ERR_ARG;
+    if (format != KTM_SEGY_IBM && format != KTM_SEGY_IEEE) return KTM_ERR_ARG;
+    int st = prep_check(p, prep_mode, dtype);
+    if (st != KTM_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    const TraceSrc src = {raw, 60 + (int64_t)p->tb.s.ns};
+    const int nt = 256;
+    hipLaunchKernelGGL(k_geometry_segy, dim3((unsigned)((ntr + nt - 1) / nt)), dim3(nt), 0, s,
+                       geo_args(p), (const uint32_t *)raw, src.rec_words, ntr, geom, coords);
+    HIP_TRY(hipGetLastError());
+    if (format == KTM_SEGY_IBM)
+        return prep_launch<SRC_SEGY_IBM>(p, src, ntr, prep_mode, dtype, prepared, geom, s);
+    return prep_launch<SRC_SEGY_IEEE>(p, src, ntr, prep_mode, dtype, prepared, geom, s);
+}
+
+/* raw SEG-Y records -> FP32 traces and header words, one lane per sample */
+template <int SRC>
+__global__ void k_segy_decode(const TraceSrc src, int64_t ntr, int ns, float *__restrict__ traces,
+                              int32_t *__restrict__ coords)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ntr * (int64_t)ns) return;
+    const int64_t i = e / ns;
+    const int t = (int)(e - i * ns);
+    if (traces) traces[e] = load_sample<SRC>(src, i, t);
+    if (coords && t < 4)
+        coords[4 * i + t] = (int32_t)__builtin_bswap32(
+            reinterpret_cast<const uint32_t *>(src.p)[i * src.rec_words + 18 + t]);
+}
+
+extern "C" int ktm_segy_decode(const void *raw, int64_t ntr, int ns, int format, float *traces,
+                               int32_t *coords, void *stream)
+{
+    if (ntr < 0 || ns < 4) return KTM_ERR_ARG;
+    if (!ntr) return KTM_OK;
+    if (!raw || ((uintptr_t)raw & 3u)) return KTM_ERR_ARG;
+    if (format != KTM_SEGY_IBM && format != KTM_SEGY_IEEE) return KTM_ERR_ARG;
+    const TraceSrc src = {raw, 60 + (int64_t)ns};
+    const int64_t n = ntr * (int64_t)ns;
+    const int nt = 256;
+    const dim3 grid((unsigned)((n + nt - 1) / nt));
+    if ((n + nt - 1) / nt > 0x7fffffffLL) return KTM_ERR_UNSUPPORTED;
+    hipStream_t s = (hipStream_t)stream;
+    if (format == KTM_SEGY_IBM)
+        hipLaunchKernelGGL(k_segy_decode<SRC_SEGY_IBM>, grid, dim3(nt), 0, s, src, ntr, ns, traces,
+                           coords);
+    else
+        hipLaunchKernelGGL(k_segy_decode<SRC_SEGY_IEEE>, grid, dim3(nt), 0, s, src, ntr, ns, traces,
+                           coords);
     HIP_TRY(hipGetLastError());
     return KTM_OK;
 }
@@ -1159,12 +1300,12 @@ static int decode_block(const ktm_plan *cp, const void *prepared, int dtype,
     for (int64_t i0 = 0; i0 < ntr; i0 += 65535) {
         const int64_t n = ntr - i0 < 65535 ? ntr - i0 : 65535;
         const dim3 grid((unsigned)((p->row + nt - 1) / nt), (unsigned)n);
-        const void *src = reinterpret_cast<const uint16_t *>(prepared) + i0 * p->row;
+        const TraceSrc src = {reinterpret_cast<const uint16_t *>(prepared) + i0 * p->row, p->row};
         if (dtype == KTM_TRACE_BF16)
-            hipLaunchKernelGGL(k_prep_closed<KTM_TRACE_BF16>, grid, dim3(nt), 0, s, src, geom + i0, n,
+            hipLaunchKernelGGL(k_prep_closed<SRC_Q_BF16>, grid, dim3(nt), 0, s, src, geom + i0, n,
                                p->tb.s.ns, p->row, p->d_w, p->d_stage + i0 * p->row);
         else
-            hipLaunchKernelGGL(k_prep_closed<KTM_TRACE_F16>, grid, dim3(nt), 0, s, src, geom + i0, n,
+            hipLaunchKernelGGL(k_prep_closed<SRC_Q_F16>, grid, dim3(nt), 0, s, src, geom + i0, n,
                                p->tb.s.ns, p->row, p->d_w, p->d_stage + i0 * p->row);
     }
     HIP_TRY(hipGetLastError());

@@ -275,19 +275,29 @@ typedef struct dev_bufs {
     int32_t *coords;
     ktm_trace_geom *geom;
     uint64_t *counter;
+    void *raw;
 } dev_bufs;
 
 static void free_bufs(dev_bufs *b)
 {
     ktm_device_free(b->traces); ktm_device_free(b->prepared); ktm_device_free(b->km);
     ktm_device_free(b->fold); ktm_device_free(b->image); ktm_device_free(b->coords);
-    ktm_device_free(b->geom); ktm_device_free(b->counter);
+    ktm_device_free(b->geom); ktm_device_free(b->counter); ktm_device_free(b->raw);
     memset(b, 0, sizeof *b);
+}
+
+/* count trace records (240-byte header + ns words each) starting at trace
+ * `first` (0-based), as stored in the file */
+static int read_records(FILE *f, int64_t first, int64_t count, int64_t rec, uint8_t *buf)
+{
+    if (fseeko(f, SEGY_FILE_HDR + first * rec, SEEK_SET) != 0) return KTM_ERR_IO;
+    return fread(buf, (size_t)rec, (size_t)count, f) == (size_t)count ? KTM_OK : KTM_ERR_IO;
 }
 
 /* One pass over all traces in blocks of ntblock (Kirchhoff.c:1276-1408),
  * for every job (job < 0) or one job.  Traces come from the host arrays
- * (traces/coords) or, when path != NULL, from the SEG-Y file. */
+ * (traces/coords) or, when path != NULL, from the SEG-Y file: each block of
+ * raw records is read as stored and decoded on the device (ktm_ingest). */
 static int run_survey(const ktm_plan *plan, const ktm_config *cfg, const char *path,
                       const float *traces, const int32_t *coords, int64_t ntr, int job,
                       dev_bufs *b, uint64_t *contrib, double *mig_ms)
@@ -297,16 +307,19 @@ static int run_survey(const ktm_plan *plan, const ktm_config *cfg, const char *p
     const int64_t blk = cfg->ntblock > 0 ? cfg->ntblock : 50000;
     const int64_t row = ktm_plan_trace_row(plan);
     const int64_t nb = ntr < blk ? ntr : blk;
+    const int64_t rec = SEGY_TRACE_HDR + 4 * (int64_t)s.ns;
     int st = KTM_OK;
-    float *htr = NULL;
-    int32_t *hco = NULL;
+    uint8_t *hraw = NULL;
+    FILE *f = NULL;
     if (path) {
-        htr = (float *)malloc(sizeof(float) * (size_t)(nb * s.ns));
-        hco = (int32_t *)malloc(sizeof(int32_t) * (size_t)(4 * nb));
-        if (!htr || !hco) st = KTM_ERR_NOMEM;
+        hraw = (uint8_t *)malloc((size_t)(nb * rec));
+        if (!hraw) st = KTM_ERR_NOMEM;
+        if (st == KTM_OK && !(f = fopen(path, "rb"))) st = KTM_ERR_IO;
+        if (st == KTM_OK) st = ktm_device_alloc(&b->raw, (size_t)(nb * rec));
+    } else {
+        st = ktm_device_alloc((void **)&b->traces, sizeof(float) * (size_t)(nb * s.ns));
+        if (st == KTM_OK) st = ktm_device_alloc((void **)&b->coords, sizeof(int32_t) * (size_t)(4 * nb));
     }
-    if (st == KTM_OK) st = ktm_device_alloc((void **)&b->traces, sizeof(float) * (size_t)(nb * s.ns));
-    if (st == KTM_OK) st = ktm_device_alloc((void **)&b->coords, sizeof(int32_t) * (size_t)(4 * nb));
     if (st == KTM_OK) st = ktm_device_alloc((void **)&b->prepared, sizeof(float) * (size_t)(nb * row));
     if (st == KTM_OK) st = ktm_device_alloc((void **)&b->geom, sizeof(ktm_trace_geom) * (size_t)nb);
     if (st == KTM_OK) st = ktm_device_alloc((void **)&b->counter, sizeof(uint64_t));
@@ -315,20 +328,28 @@ static int run_survey(const ktm_plan *plan, const ktm_config *cfg, const char *p
     double t_mig = 0.0;
     for (int64_t i0 = 0; i0 < ntr && st == KTM_OK; i0 += blk) {
         const int64_t n = ntr - i0 < blk ? ntr - i0 : blk;
-        const float *src_tr = traces + i0 * s.ns;
-        const int32_t *src_co = coords + 4 * i0;
         if (path) {
-            st = ktm_segy_read(path, i0, n, 0, htr, hco, NULL);
-            src_tr = htr;
-            src_co = hco;
+            st = read_records(f, i0, n, rec, hraw);
+            /* header words 18-21 (Kirchhoff.c:1297,1302): htonl() - ox on a
+             * negative word is undefined in the reference */
+            for (int64_t k = 0; k < n && st == KTM_OK; ++k)
+                for (int c = 0; c < 4; ++c)
+                    if (hraw[k * rec + 72 + 4 * c] & 0x80u) st = KTM_ERR_ARG;
+            if (st == KTM_OK) st = ktm_copy_to_device(b->raw, hraw, (size_t)(n * rec));
+            if (st == KTM_OK)
+                st = ktm_ingest(plan, b->raw, n, KTM_SEGY_IBM, KTM_PREP_REFERENCE, KTM_TRACE_F32,
+                                b->prepared, b->geom, NULL, NULL);
+        } else {
+            const float *src_tr = traces + i0 * s.ns;
+            const int32_t *src_co = coords + 4 * i0;
+            for (int64_t k = 0; k < 4 * n && st == KTM_OK; ++k)
+                if (src_co[k] < 0) st = KTM_ERR_ARG; /* htonl() - ox on a negative word is undefined */
+            if (st == KTM_OK) st = ktm_copy_to_device(b->traces, src_tr, sizeof(float) * (size_t)(n * s.ns));
+            if (st == KTM_OK) st = ktm_copy_to_device(b->coords, src_co, sizeof(int32_t) * (size_t)(4 * n));
+            if (st == KTM_OK)
+                st = ktm_prep(plan, b->traces, b->coords, n, KTM_PREP_REFERENCE, KTM_TRACE_F32,
+                              b->prepared, b->geom, NULL);
         }
-        for (int64_t k = 0; k < 4 * n && st == KTM_OK; ++k)
-            if (src_co[k] < 0) st = KTM_ERR_ARG; /* htonl() - ox on a negative word is undefined */
-        if (st == KTM_OK) st = ktm_copy_to_device(b->traces, src_tr, sizeof(float) * (size_t)(n * s.ns));
-        if (st == KTM_OK) st = ktm_copy_to_device(b->coords, src_co, sizeof(int32_t) * (size_t)(4 * n));
-        if (st == KTM_OK)
-            st = ktm_prep(plan, b->traces, b->coords, n, KTM_PREP_REFERENCE, KTM_TRACE_F32,
-                          b->prepared, b->geom, NULL);
         const double t0 = now_ms();
         if (st == KTM_OK)
             st = ktm_migrate_block(plan, b->prepared, KTM_TRACE_F32, KTM_PREP_REFERENCE, b->geom, n,
@@ -338,8 +359,8 @@ static int run_survey(const ktm_plan *plan, const ktm_config *cfg, const char *p
     }
     if (st == KTM_OK) st = ktm_copy_to_host(contrib, b->counter, sizeof(uint64_t));
     *mig_ms = t_mig;
-    free(htr);
-    free(hco);
+    if (f) fclose(f);
+    free(hraw);
     return st;
 }
 

@@ -17,8 +17,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import (ACCUM_KAHAN, ACCUM_PLAIN, PREP_REFERENCE, TRACE_F32, KtmConfig, KtmSetup,
-                   KtmStats, check, lib)
+from ._lib import (ACCUM_KAHAN, ACCUM_PLAIN, PREP_REFERENCE, SEGY_IBM, TRACE_F32, KtmConfig,
+                   KtmSetup, KtmStats, check, lib)
 
 
 @dataclass
@@ -156,6 +156,22 @@ class Plan:
                              _ptr(geom), _stream()), "ktm_prep")
         return prepared, geom
 
+    def ingest(self, raw, ntr: int, fmt: int = SEGY_IBM, mode: int = PREP_REFERENCE,
+               dtype: int = TRACE_F32):
+        """Prepared traces, geometry and header words from ``ntr`` raw SEG-Y
+        trace records (uint8 device tensor, 240 + 4 ns bytes each, as stored
+        in the file), decoded on the device (``ktm_ingest``)."""
+        import torch
+        rec = 240 + 4 * self.ns
+        assert raw.dtype == torch.uint8 and raw.is_contiguous() and raw.numel() >= ntr * rec
+        et = torch.float32 if dtype == TRACE_F32 else torch.int16
+        prepared = torch.empty((ntr, self.row), dtype=et, device=raw.device)
+        geom = torch.empty((ntr, 8), dtype=torch.float32, device=raw.device)
+        coords = torch.empty((ntr, 4), dtype=torch.int32, device=raw.device)
+        check(lib().ktm_ingest(self._h, _ptr(raw), ntr, fmt, mode, dtype, _ptr(prepared),
+                               _ptr(geom), _ptr(coords), _stream()), "ktm_ingest")
+        return prepared, geom, coords
+
     def migrate_block(self, prepared, geom, km, fold, comp=None, *, dtype: int = TRACE_F32,
                       mode: int = PREP_REFERENCE, accum: int = ACCUM_PLAIN, job: int = -1,
                       counter=None) -> None:
@@ -215,6 +231,17 @@ class Plan:
         return km, fold, int(counter.item())
 
 
+def segy_decode(raw, ntr: int, ns: int, fmt: int = SEGY_IBM):
+    """(traces float32 [ntr][ns], header words int32 [ntr][4]) of raw SEG-Y
+    trace records on the device (``ktm_segy_decode``)."""
+    import torch
+    traces = torch.empty((ntr, ns), dtype=torch.float32, device=raw.device)
+    coords = torch.empty((ntr, 4), dtype=torch.int32, device=raw.device)
+    check(lib().ktm_segy_decode(_ptr(raw), ntr, ns, fmt, _ptr(traces), _ptr(coords), _stream()),
+          "ktm_segy_decode")
+    return traces, coords
+
+
 def migrate_host(cfg: Config, ns: int, dt: float, traces: np.ndarray, coords: np.ndarray,
                  vel: np.ndarray, dtm: float | None = None):
     """``ktm_migrate`` on host arrays: returns (km, fold, image, stats dict)."""
@@ -236,4 +263,4 @@ def migrate_host(cfg: Config, ns: int, dt: float, traces: np.ndarray, coords: np
                              "total_ms": st.total_ms}
 
 
-__all__ = ["Config", "Plan", "migrate_host", "setup_info", "ACCUM_KAHAN", "_lib"]
+__all__ = ["Config", "Plan", "migrate_host", "segy_decode", "setup_info", "ACCUM_KAHAN", "_lib"]
