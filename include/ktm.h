@@ -159,6 +159,14 @@ int ktm_ingest(const ktm_plan *plan, const void *raw, int64_t ntr, int format,
 int ktm_segy_decode(const void *raw, int64_t ntr, int ns, int format, float *traces,
                     int32_t *coords, void *stream);
 
+/* Encode FP32 traces [ntr][ns] (device) into raw SEG-Y trace records
+ * (device, ntr * (240 + 4 ns) bytes, 4-byte aligned): the 240-byte headers
+ * copied from headers ([ntr][240] device bytes, or zeros if NULL) and the
+ * samples as big-endian IEEE (format 5) or IBM (format 1, truncating, the
+ * same encoding as ktm_segy_write). */
+int ktm_segy_encode(const float *traces, int64_t ntr, int ns, int format, const void *headers,
+                    void *records, void *stream);
+
 /* km = 0, fold = 1, comp = 0 (Kirchhoff.c:1262-1274); comp may be NULL. */
 int ktm_image_reset(const ktm_plan *plan, float *km, float *fold, float *comp,
                     void *stream);
@@ -242,6 +250,15 @@ int ktm_vgrad_velocity(ktm_vgrad *vg, const float *V, float *v, void *stream);
  * I [nxp][nh][ns] (device).  Synchronises the stream. */
 int ktm_vgrad_eval(ktm_vgrad *vg, const float *V, double *J, double *dJdV, float *cig,
                    void *stream);
+
+/* The trace records of image<job>.su (Kirchhoff.c:1437-1598) on the device:
+ * for every (xc, yc, ih) of the job's slab in the reference's order, the
+ * 240-byte header of ktm_segy_image_trace_header and ns big-endian IEEE
+ * samples of image [nxp][nyp][nh][ns] (the ktm_poststack output).  records:
+ * device, nx*ny*nh*(240 + 4 ns) bytes, 4-byte aligned.  The file is these
+ * bytes written as they are. */
+int ktm_image_records(const ktm_plan *plan, const float *image, int job, void *records,
+                      void *stream);
 
 /* Host convenience: whole reference job loop for all jobs (Kirchhoff.c:
  * 347-350 on one rank): prep + migrate in blocks of cfg->ntblock + post-stack.

@@ -115,6 +115,12 @@ _PROTOS = {
     "ktm_poststack_adjoint": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "ktm_ingest": (_i32, [_vp, _vp, _i64, _i32, _i32, _i32, _vp, _vp, _vp, _vp]),
     "ktm_segy_decode": (_i32, [_vp, _i64, _i32, _i32, _vp, _vp, _vp]),
+    "ktm_segy_encode": (_i32, [_vp, _i64, _i32, _i32, _vp, _vp, _vp]),
+    "ktm_image_records": (_i32, [_vp, _vp, _i32, _vp, _vp]),
+    "ktm_device_alloc": (_i32, [C.POINTER(_vp), C.c_size_t]),
+    "ktm_device_free": (_i32, [_vp]),
+    "ktm_copy_to_device": (_i32, [_vp, _vp, C.c_size_t]),
+    "ktm_copy_to_host": (_i32, [_vp, _vp, C.c_size_t]),
     "ktm_vgrad_create": (_i32, [C.POINTER(_vp), C.POINTER(KtmConfig), _i32, _f32, _i32, _i32, _i32]),
     "ktm_vgrad_destroy": (_i32, [_vp]),
     "ktm_vgrad_bind": (_i32, [_vp, _vp, _i64, _vp, _i64, _vp]),

@@ -17,6 +17,8 @@
  *                   in registers), streams the traces of a block through LDS in
  *                   trace order and accumulates F(floor(dsr/dt)) per sample
  *   k_poststack     derivative/fold + low-pass, Kirchhoff.c:1457-1479
+ *   k_image_*       image<job>.su trace records (headers + big-endian
+ *                   samples), Kirchhoff.c:1437-1598
  *   k_demigrate     exact transpose of k_migrate (CLOSED prep)
  *   k_synth_*       seeded synthetic surveys
  *
@@ -1245,6 +1247,179 @@ extern "C" int ktm_segy_decode(const void *raw, int64_t ntr, int ns, int format,
     else
         hipLaunchKernelGGL(k_segy_decode<SRC_SEGY_IEEE>, grid, dim3(nt), 0, s, src, ntr, ns, traces,
                            coords);
+    HIP_TRY(hipGetLastError());
+    return KTM_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* SEG-Y output on the device                                                */
+/* ------------------------------------------------------------------------ */
+
+/* IEEE -> IBM hex float, truncating (the encoder of the host's SEG-Y
+ * writer): v = M 2^s with a 24-bit M (top bit 23), e2 = s + 24 (frexp's
+ * exponent), e16 = ceil(e2 / 4), mantissa M >> (4 e16 - e2).  Zero, NaN and
+ * inf encode as 0. */
+__device__ __forceinline__ uint32_t float_to_ibm(float v)
+{
+    const uint32_t b = __float_as_uint(v);
+    const uint32_t E = (b >> 23) & 0xffu, F = b & 0x7fffffu;
+    if (E == 0xffu || (E == 0u && F == 0u)) return 0u;
+    uint32_t M;
+    int e2;
+    if (E) {
+        M = F | 0x800000u;
+        e2 = (int)E - 126;
+    } else {                                /* subnormal: normalise */
+        const int sh = __builtin_clz(F) - 8;
+        M = F << sh;
+        e2 = -125 - sh;
+    }
+    const int e16 = (e2 + 3) >> 2;
+    const uint32_t mant = M >> (4 * e16 - e2);
+    return (b & 0x80000000u) | ((uint32_t)(e16 + 64) << 24) | mant;
+}
+
+/* samples of ntr records: records[r][60 + t] = big-endian word of traces[r][t] */
+template <int FMT>
+__global__ void k_encode_samples(const float *__restrict__ traces, int64_t ntr, int ns,
+                                 uint32_t *__restrict__ rec)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ntr * (int64_t)ns) return;
+    const int64_t r = e / ns;
+    const int t = (int)(e - r * ns);
+    const float v = traces[e];
+    const uint32_t w = FMT == KTM_SEGY_IBM ? float_to_ibm(v) : __float_as_uint(v);
+    rec[r * (60 + (int64_t)ns) + 60 + t] = __builtin_bswap32(w);
+}
+
+/* headers [ntr][60 words] into the records (or zeros) */
+__global__ void k_copy_headers(const uint32_t *__restrict__ hdr, int64_t ntr, int ns,
+                               uint32_t *__restrict__ rec)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ntr * 60) return;
+    const int64_t r = e / 60;
+    rec[r * (60 + (int64_t)ns) + (e - r * 60)] = hdr ? hdr[e] : 0u;
+}
+
+struct ImgHdrArgs {
+    double start_il, start_xl, inc_il, inc_xl, il_dist, xl_dist, oh, dh;
+    int ox_i, oy_i;                 /* (int)ox, (int)oy of Kirchhoff.c:1492-1493 */
+    float cosA, sinA;
+    int nx, ny, nyp, Nx, nh, ns, job, jobs_y;
+    uint32_t dt_us;                 /* (unsigned short)(dt * 1e6), Kirchhoff.c:1447 */
+};
+
+/* image<job>.su, Kirchhoff.c:1437-1598: record r = (xc ny + yc) nh + ih of
+ * the job's slab; one lane per record writes its 240-byte header (the
+ * fields the reference sets, big-endian, the rest zero) */
+__global__ void k_image_headers(const ImgHdrArgs a, uint32_t *__restrict__ rec)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nrec = (int64_t)a.nx * a.ny * a.nh;
+    if (r >= nrec) return;
+    const int ih = (int)(r % a.nh);
+    const int64_t c = r / a.nh;
+    const int xc = (int)(c / a.ny), yc = (int)(c % a.ny);
+    const int X = xc + (a.job / a.jobs_y) * a.nx;
+    const int Y = yc + (a.job % a.jobs_y) * a.ny;
+    const double il = a.start_il + X * a.inc_il;                 /* Kirchhoff.c:1482 */
+    const double xl = a.start_xl + Y * a.inc_xl;                 /* Kirchhoff.c:1483 */
+    const int xm = (int)(il * a.il_dist);                        /* Kirchhoff.c:1485 */
+    const int ym = (int)(xl * a.xl_dist);                        /* Kirchhoff.c:1486 */
+    const float cx = (float)a.ox_i + a.cosA * (float)xm - a.sinA * (float)ym;
+    const float cy = (float)a.oy_i + a.sinA * (float)xm + a.cosA * (float)ym;
+    const int tracl = a.nx * a.ny * a.nh * a.job + 1 + (xc * a.ny + yc) * a.nh + ih;
+    uint32_t *h = rec + r * (60 + (int64_t)a.ns);
+    for (int w = 0; w < 60; ++w) {
+        uint32_t v = 0u;
+        switch (w) {
+        case 0: v = __builtin_bswap32((uint32_t)tracl); break;                  /* tracl */
+        case 5: v = __builtin_bswap32((uint32_t)(X + a.Nx * Y + 1)); break;     /* cdp */
+        case 9: v = __builtin_bswap32((uint32_t)(int)ceil(a.oh + ih * a.dh)); break;  /* offset */
+        case 18: case 19: case 20: case 21: v = __builtin_bswap32(1u); break;   /* sx..gy */
+        case 28: v = __builtin_bswap32((uint32_t)(a.ns & 0xffff)); break;       /* ns, 114 */
+        case 29: v = __builtin_bswap32(a.dt_us << 16); break;                   /* dt, 116 */
+        case 45: v = __builtin_bswap32((uint32_t)(int)cx); break;               /* cdpx */
+        case 46: v = __builtin_bswap32((uint32_t)(int)cy); break;               /* cdpy */
+        case 47: v = __builtin_bswap32((uint32_t)(int)il); break;               /* iline */
+        case 48: v = __builtin_bswap32((uint32_t)(int)xl); break;               /* xline */
+        case 59: v = __builtin_bswap32(9999u); break;                           /* :1589-1590 */
+        default: break;
+        }
+        h[w] = v;
+    }
+}
+
+/* samples of image<job>.su: record r of the slab <- image[X][Y][ih][t] */
+__global__ void k_image_samples(const float *__restrict__ image, const ImgHdrArgs a,
+                                uint32_t *__restrict__ rec)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nrec = (int64_t)a.nx * a.ny * a.nh;
+    if (e >= nrec * a.ns) return;
+    const int64_t r = e / a.ns;
+    const int t = (int)(e - r * a.ns);
+    const int ih = (int)(r % a.nh);
+    const int64_t c = r / a.nh;
+    const int64_t X = c / a.ny + (int64_t)(a.job / a.jobs_y) * a.nx;
+    const int64_t Y = c % a.ny + (int64_t)(a.job % a.jobs_y) * a.ny;
+    const float v = image[((X * a.nyp + Y) * a.nh + ih) * a.ns + t];
+    rec[r * (60 + (int64_t)a.ns) + 60 + t] = __builtin_bswap32(__float_as_uint(v));
+}
+
+extern "C" int ktm_segy_encode(const float *traces, int64_t ntr, int ns, int format,
+                               const void *headers, void *records, void *stream)
+{
+    if (ntr < 0 || ns < 1) return KTM_ERR_ARG;
+    if (!ntr) return KTM_OK;
+    if (!traces || !records || ((uintptr_t)records & 3u) || ((uintptr_t)headers & 3u))
+        return KTM_ERR_ARG;
+    if (format != KTM_SEGY_IBM && format != KTM_SEGY_IEEE) return KTM_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const int nt = 256;
+    const int64_t n = ntr * (int64_t)ns;
+    if ((n + nt - 1) / nt > 0x7fffffffLL || (ntr * 60 + nt - 1) / nt > 0x7fffffffLL)
+        return KTM_ERR_UNSUPPORTED;
+    hipLaunchKernelGGL(k_copy_headers, dim3((unsigned)((ntr * 60 + nt - 1) / nt)), dim3(nt), 0, s,
+                       (const uint32_t *)headers, ntr, ns, (uint32_t *)records);
+    if (format == KTM_SEGY_IBM)
+        hipLaunchKernelGGL(k_encode_samples<KTM_SEGY_IBM>, dim3((unsigned)((n + nt - 1) / nt)),
+                           dim3(nt), 0, s, traces, ntr, ns, (uint32_t *)records);
+    else
+        hipLaunchKernelGGL(k_encode_samples<KTM_SEGY_IEEE>, dim3((unsigned)((n + nt - 1) / nt)),
+                           dim3(nt), 0, s, traces, ntr, ns, (uint32_t *)records);
+    HIP_TRY(hipGetLastError());
+    return KTM_OK;
+}
+
+extern "C" int ktm_image_records(const ktm_plan *p, const float *image, int job, void *records,
+                                 void *stream)
+{
+    if (!p || !image || !records || ((uintptr_t)records & 3u)) return KTM_ERR_ARG;
+    const ktm_setup *st = &p->tb.s;
+    if (job < 0 || job >= st->no_job) return KTM_ERR_ARG;
+    const ktm_config *c = &p->cfg;
+    ImgHdrArgs a;
+    a.start_il = c->start_iline; a.start_xl = c->start_xline;
+    a.inc_il = c->inc_iline; a.inc_xl = c->inc_xline;
+    a.il_dist = c->iline_distance; a.xl_dist = c->xline_distance;
+    a.oh = c->oh; a.dh = c->dh;
+    a.ox_i = (int)c->ox; a.oy_i = (int)c->oy;
+    a.cosA = st->cosA; a.sinA = st->sinA;
+    a.nx = st->nx; a.ny = st->ny; a.nyp = st->nyp; a.Nx = st->Nx; a.nh = st->nh; a.ns = st->ns;
+    a.job = job; a.jobs_y = c->jobs_y;
+    a.dt_us = (uint32_t)(uint16_t)(st->dt * 1000000);
+    const int64_t nrec = (int64_t)a.nx * a.ny * a.nh;
+    const int64_t n = nrec * a.ns;
+    const int nt = 256;
+    if ((n + nt - 1) / nt > 0x7fffffffLL) return KTM_ERR_UNSUPPORTED;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_image_headers, dim3((unsigned)((nrec + nt - 1) / nt)), dim3(nt), 0, s, a,
+                       (uint32_t *)records);
+    hipLaunchKernelGGL(k_image_samples, dim3((unsigned)((n + nt - 1) / nt)), dim3(nt), 0, s, image,
+                       a, (uint32_t *)records);
     HIP_TRY(hipGetLastError());
     return KTM_OK;
 }

@@ -486,40 +486,26 @@ static int kirchhoff_job(int job_id)
     if (st == KTM_OK)
         st = run_survey(G.plan, &G.cfg, G.input_data, NULL, NULL, G.nt, job_id, &b, &contrib, &ms);
     if (st == KTM_OK) st = ktm_poststack(G.plan, b.km, b.fold, b.image, NULL);
-    float *img = NULL;
-    if (st == KTM_OK) {
-        img = (float *)malloc(sizeof(float) * nimg);
-        if (!img) st = KTM_ERR_NOMEM;
-    }
-    if (st == KTM_OK) st = ktm_copy_to_host(img, b.image, sizeof(float) * nimg);
-    free_bufs(&b);
-    if (st != KTM_OK) { free(img); return st; }
     /* image<job>.su: per (xc, yc, ih) a 240-byte header and ns big-endian
-     * IEEE floats, in the order of Kirchhoff.c:1440-1598 */
+     * IEEE floats, in the order of Kirchhoff.c:1440-1598, encoded on the
+     * device and written as one block */
+    const size_t nrec = (size_t)s.nx * s.ny * s.nh;
+    const size_t bytes = nrec * (SEGY_TRACE_HDR + 4 * (size_t)s.ns);
+    void *drec = NULL;
+    uint8_t *hrec = NULL;
+    if (st == KTM_OK) st = ktm_device_alloc(&drec, bytes);
+    if (st == KTM_OK) st = ktm_image_records(G.plan, b.image, job_id, drec, NULL);
+    if (st == KTM_OK && !(hrec = (uint8_t *)malloc(bytes))) st = KTM_ERR_NOMEM;
+    if (st == KTM_OK) st = ktm_copy_to_host(hrec, drec, bytes);
+    ktm_device_free(drec);
+    free_bufs(&b);
+    if (st != KTM_OK) { free(hrec); return st; }
     char name[4200];
     snprintf(name, sizeof name, "%s/image%d.su", G.image_dir, job_id);
     FILE *f = fopen(name, "wb");
-    if (!f) { free(img); return KTM_ERR_IO; }
-    const int xs = job_id / G.cfg.jobs_y, ys = job_id % G.cfg.jobs_y;
-    uint8_t hdr[240];
-    uint8_t *data = (uint8_t *)malloc((size_t)s.ns * 4);
-    if (!data) st = KTM_ERR_NOMEM;
-    for (int xc = 0; xc < s.nx && st == KTM_OK; ++xc)
-        for (int yc = 0; yc < s.ny && st == KTM_OK; ++yc)
-            for (int ih = 0; ih < s.nh && st == KTM_OK; ++ih) {
-                const size_t X = (size_t)xs * s.nx + xc, Y = (size_t)ys * s.ny + yc;
-                const float *tr = img + ((X * s.nyp + Y) * s.nh + ih) * s.ns;
-                ktm_segy_image_trace_header(&G.cfg, s.ns, s.dt, job_id, xc, yc, ih, hdr);
-                for (int t = 0; t < s.ns; ++t) {
-                    uint32_t u;
-                    memcpy(&u, tr + t, 4);
-                    ktm_put_be32(data + 4 * t, u);
-                }
-                if (fwrite(hdr, 1, 240, f) != 240 || fwrite(data, 4, (size_t)s.ns, f) != (size_t)s.ns)
-                    st = KTM_ERR_IO;
-            }
-    free(data);
-    free(img);
+    if (!f) { free(hrec); return KTM_ERR_IO; }
+    if (fwrite(hrec, 1, bytes, f) != bytes) st = KTM_ERR_IO;
+    free(hrec);
     if (fclose(f) != 0 && st == KTM_OK) st = KTM_ERR_IO;
     return st;
 }

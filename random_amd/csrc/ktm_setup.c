@@ -77,6 +77,7 @@ int ktm_setup_init(const ktm_config *c, int ns, float dt, int nsm, float dtm,
     s->ns = ns; s->nsm = nsm; s->dt = dt; s->dtm = dtm;
     s->Nx = cells_along(c->start_iline, c->end_iline, c->inc_iline);
     s->Ny = cells_along(c->start_xline, c->end_xline, c->inc_xline);
+    if (s->Nx < 0 || s->Ny < 0) return KTM_ERR_ARG;   /* end line before start line */
     s->no_job = c->jobs_x * c->jobs_y;
     s->nx = (int)ceil(1.0 * s->Nx / c->jobs_x);
     s->ny = (int)ceil(1.0 * s->Ny / c->jobs_y);

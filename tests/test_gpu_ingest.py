@@ -104,3 +104,76 @@ def test_ingest_rejects_bad_arguments(cuda):
         plan.ingest(raw, 2, 3)                             # format 3 (int16) unsupported
     with pytest.raises(KtmError):
         plan.ingest(raw, 2, SEGY_IBM, PREP_REFERENCE, TRACE_BF16)
+
+
+def _segy_encode(traces, fmt, headers=None):
+    ntr, ns = traces.shape
+    rec = torch.empty(ntr * (240 + 4 * ns), dtype=torch.uint8, device=traces.device)
+    _lib.check(_lib.lib().ktm_segy_encode(traces.data_ptr(), ntr, ns, fmt,
+                                          None if headers is None else headers.data_ptr(),
+                                          rec.data_ptr(), torch.cuda.current_stream().cuda_stream),
+               "ktm_segy_encode")
+    return rec
+
+
+@pytest.mark.parametrize("ieee", [False, True], ids=["ibm", "ieee"])
+def test_encode_matches_host_writer(cuda, tmp_path, ieee):
+    print(f"case: device SEG-Y encode, ieee={ieee}", flush=True)
+    rng = np.random.default_rng(11)
+    ntr, ns = 37, 513
+    x = (rng.standard_normal((ntr, ns)) * 10.0 ** rng.integers(-30, 30, (ntr, ns))).astype(np.float32)
+    x[0, :8] = [0.0, -0.0, np.inf, -np.inf, np.nan, 1e-45, -3e-39, np.finfo(np.float32).max]
+    hdr = rng.integers(0, 256, (ntr, 240), dtype=np.uint8)
+    fh = np.zeros(3600, np.uint8)
+    path = str(tmp_path / "out.sgy")
+    _lib.check(_lib.lib().ktm_segy_write(path.encode(), fh.ctypes.data_as(C.c_void_p),
+                                         hdr.ctypes.data_as(C.c_void_p),
+                                         x.ctypes.data_as(C.c_void_p), ntr, ns, int(ieee)),
+               "ktm_segy_write")
+    want = np.fromfile(path, np.uint8)[3600:]
+    got = _segy_encode(torch.from_numpy(x).to(cuda), SEGY_IEEE if ieee else SEGY_IBM,
+                       torch.from_numpy(hdr).to(cuda)).cpu().numpy()
+    assert np.array_equal(got, want)
+    if not ieee:
+        # the words decode (oracle) to the value truncated to a 21-24 bit hex
+        # mantissa; the decode's own power of 16 overflows float at 16^32,
+        # so the largest floats are left out
+        dec = oracle.ibm_to_float(got.reshape(ntr, -1)[:, 240:].ravel()).reshape(ntr, ns)
+        fin = np.isfinite(x) & (np.abs(x) < 2.0 ** 120)
+        xf = x[fin].astype(np.float64)
+        rel = np.abs(dec[fin].astype(np.float64) - xf) / np.maximum(np.abs(xf), 1e-300)
+        assert rel.max() < 2.0 ** -20
+
+
+def test_image_records_match_host_headers(cuda):
+    print("case: image records vs host headers", flush=True)
+    sv = surveys.area3d(ntr=300, ns=257, ncols=12, extent=600.0, max_offset=900.0, nscat=2,
+                        jobs_x=2, jobs_y=3)
+    cfg = sv.cfg.replace(angle_e_to_n=-10.369, ox=444479.29, oy=478003.98, nh=3, dh=300.0,
+                         oh=50.0, start_iline=1500, end_iline=1512, start_xline=2500,
+                         end_xline=2512, ovel_ilines=1500, ovel_xlines=2500, image_origin_zero=1)
+    vel = np.full((cfg.nvel_ilines, cfg.nvel_xlines, sv.ns), 2000.0, np.float32)
+    plan = Plan(cfg, sv.ns, sv.dt, vel)
+    rng = np.random.default_rng(4)
+    image = torch.from_numpy(rng.standard_normal(plan.image_shape).astype(np.float32)).to(cuda)
+    img = image.cpu().numpy()
+    L = _lib.lib()
+    c = cfg.to_c()
+    nx, ny, nh, ns = plan.setup["nx"], plan.setup["ny"], plan.nh, plan.ns
+    for job in range(plan.setup["no_job"]):
+        rec = torch.empty(nx * ny * nh * (240 + 4 * ns), dtype=torch.uint8, device=cuda)
+        _lib.check(L.ktm_image_records(plan._h, image.data_ptr(), job, rec.data_ptr(),
+                                       torch.cuda.current_stream().cuda_stream), "ktm_image_records")
+        got = rec.cpu().numpy().reshape(nx * ny * nh, 240 + 4 * ns)
+        xs, ys = job // cfg.jobs_y, job % cfg.jobs_y
+        r = 0
+        for xc in range(nx):
+            for yc in range(ny):
+                for ih in range(nh):
+                    h = np.zeros(240, np.uint8)
+                    L.ktm_segy_image_trace_header(C.byref(c), ns, plan.dt, job, xc, yc, ih,
+                                                  h.ctypes.data_as(C.c_void_p))
+                    assert np.array_equal(got[r, :240], h), (job, xc, yc, ih)
+                    tr = img[xs * nx + xc, ys * ny + yc, ih].astype(">f4").view(np.uint8)
+                    assert np.array_equal(got[r, 240:], tr)
+                    r += 1

@@ -40,6 +40,12 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_every_declared_function_has_a_ctypes_prototype():
+    # an unprototyped ctypes call passes int64 arguments as 32-bit ints
+    missing = [n for n in header_names() if n not in _lib._PROTOS]
+    assert not missing, missing
+
+
 def test_oracle_is_a_separate_library():
     # the product library does not contain (or link) the oracle
     L = _lib.lib()
@@ -193,4 +199,7 @@ def test_setup_errors():
     assert e.value.status == _lib.KTM_ERR_UNSUPPORTED
     with pytest.raises(_lib.KtmError) as e:
         setup_info(Config(jobs_x=0), 1001, np.float32(0.004))
+    assert e.value.status == _lib.KTM_ERR_ARG
+    with pytest.raises(_lib.KtmError) as e:
+        setup_info(Config(start_iline=1600, end_iline=1500), 1001, np.float32(0.004))
     assert e.value.status == _lib.KTM_ERR_ARG
