@@ -54,6 +54,7 @@ def parse():
     ap.add_argument("--ncols", type=int, default=400)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline time")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the ingest and velocity-gradient legs")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3")
     ap.add_argument("--leg-steps", type=int, default=3, help="timed blocks per precision leg")
     ap.add_argument("--err-traces", type=int, default=20000,
@@ -151,6 +152,103 @@ def precision_errors(sv, plan, traces, coords, ntr, dev):
         out.setdefault(label, {}).update(e)
         log(f"{label}: {e}")
     return out
+
+
+def ingest_leg(sv, plan, traces, block, dev, reps=5):
+    """Device SEG-Y ingest of one block of raw IBM trace records (as read from
+    a file, Kirchhoff.c:1281-1290): fused byte swap + IBM decode + header
+    words + FP32 reference prep (ktm_ingest), timed beside the decode alone
+    and the prep of already-decoded traces."""
+    import numpy as np
+    import torch
+
+    from random_amd import _lib
+    from random_amd._lib import PREP_REFERENCE, SEGY_IBM, TRACE_F32
+    from random_amd.ktm import segy_decode
+    n, ns = min(block, traces.shape[0]), plan.ns
+    rec = 240 + 4 * ns
+    hdr = np.zeros((n, 240), np.uint8)
+    hdr[:, 72:88] = sv.coords[:n].astype(">i4").view(np.uint8).reshape(n, 16)
+    d_hdr = torch.from_numpy(hdr).to(dev)
+    raw = torch.empty(n * rec, dtype=torch.uint8, device=dev)
+    _lib.check(_lib.lib().ktm_segy_encode(traces[:n].data_ptr(), n, ns, SEGY_IBM, d_hdr.data_ptr(),
+                                          raw.data_ptr(), torch.cuda.current_stream().cuda_stream),
+               "ktm_segy_encode")
+    dec, co = segy_decode(raw, n, ns, SEGY_IBM)
+
+    def timed(fn):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    ms_ingest = timed(lambda: plan.ingest(raw, n, SEGY_IBM, PREP_REFERENCE, TRACE_F32))
+    ms_decode = timed(lambda: segy_decode(raw, n, ns, SEGY_IBM))
+    ms_prep = timed(lambda: plan.prep(dec, co, PREP_REFERENCE, TRACE_F32))
+    b_ingest = n * (rec + 4 * plan.row + 32)          # records in, prepared rows + geometry out
+    b_decode = n * (rec + 4 * ns + 16)
+    b_prep = n * (4 * ns + 16 + 4 * plan.row + 32)
+    out = {"traces": n, "record_bytes": rec, "format": "IBM (SEG-Y format 1)"}
+    for k, ms, b in (("ingest_fused", ms_ingest, b_ingest), ("decode_only", ms_decode, b_decode),
+                     ("prep_from_fp32", ms_prep, b_prep)):
+        gbs = b / (ms * 1e-3) / 1e9
+        out[k] = {"ms": round(ms, 3), "bytes": int(b), "GB_per_s": round(gbs, 1),
+                  "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+        log(f"ingest {k}: {ms:.3f} ms, {gbs:.0f} GB/s")
+    return out
+
+
+def vgrad_leg(dev, reps=3, fd_nodes=10):
+    """Velocity gradient of the gather-flatness objective on the 600 x 240
+    line (2001 samples, 2000 columns x 64 offset classes, 100 x 200 spline
+    nodes, time chunks of 250 samples): objective alone, objective +
+    gradient, and central differences (h = 1 m/s) at seeded nodes."""
+    import numpy as np
+    import torch
+
+    from random_amd import surveys, synth
+    from random_amd.ktm import Plan
+    from random_amd.vgrad import VelocityGradient, central_differences, linear_nodes
+    sv = surveys.line2d_vgrad()
+    plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
+    tr, co = synth.traces_gpu(plan, sv, device=dev)
+    vg = VelocityGradient(plan, tr, co, nt_nodes=100, nx_nodes=200, chunk=250)
+    V = linear_nodes(sv.ns, sv.dt, 100, 200, 1500.0, 400.0, scale=0.97)
+    vg(V)
+
+    def timed(grad):
+        t = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            vg(V, grad=grad)
+            t.append((time.perf_counter() - t0) * 1e3)
+        return float(np.median(t))
+
+    ms_j, ms_g = timed(False), timed(True)
+    per_pass = vg.pairs * sv.ns
+    J, g, _ = vg(V)
+    g = g.cpu().numpy()
+    rng = np.random.default_rng(20240042)
+    nodes = [(int(rng.integers(2, 98)), int(rng.integers(2, 198))) for _ in range(fd_nodes)]
+    fd = central_differences(vg, V, nodes, h=1.0)
+    ad = np.array([g[i, j] for i, j in nodes])
+    rel = np.abs(ad - fd) / np.abs(ad)
+    log(f"vgrad: J {ms_j:.1f} ms, J+grad {ms_g:.1f} ms, {per_pass:.3e} contributions per pass, "
+        f"FD max rel {rel.max():.2e}")
+    return {"workload": "2-D line 600 shots x 240 receivers (144,000 traces), 2001 samples, "
+                        "2000 columns x 64 offset classes, v on 100 x 200 B-spline nodes, "
+                        "time chunks of 250 samples",
+            "contributions_per_pass": int(per_pass), "ms_objective": round(ms_j, 1),
+            "ms_objective_and_gradient": round(ms_g, 1),
+            "G_contributions_per_s_objective": round(per_pass / (ms_j * 1e-3) / 1e9, 1),
+            "fd_nodes": fd_nodes, "fd_h_m_per_s": 1.0,
+            "fd_max_rel_err": float(rel.max()), "fd_median_rel_err": float(np.median(rel))}
 
 
 def main():
@@ -264,6 +362,9 @@ def main():
                 else:
                     legs[k] = v
         result["mixed_precision"] = legs
+    if not args.profile and not args.no_extra:
+        result["ingest"] = ingest_leg(sv, plan, traces, args.block, dev)
+        result["velocity_gradient"] = vgrad_leg(dev)
     if not args.no_cpu and not args.profile:
         nthreads = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
         nthreads = max(1, min(nthreads, 64))

@@ -250,6 +250,9 @@ int ktm_vgrad_velocity(ktm_vgrad *vg, const float *V, float *v, void *stream);
  * I [nxp][nh][ns] (device).  Synchronises the stream. */
 int ktm_vgrad_eval(ktm_vgrad *vg, const float *V, double *J, double *dJdV, float *cig,
                    void *stream);
+/* (trace, column) pairs of the bound block that pass the aperture gate (each
+ * contributes ns samples per pass); -1 before ktm_vgrad_bind. */
+int64_t ktm_vgrad_pairs(const ktm_vgrad *vg);
 
 /* The trace records of image<job>.su (Kirchhoff.c:1437-1598) on the device:
  * for every (xc, yc, ih) of the job's slab in the reference's order, the

@@ -126,6 +126,7 @@ _PROTOS = {
     "ktm_vgrad_bind": (_i32, [_vp, _vp, _i64, _vp, _i64, _vp]),
     "ktm_vgrad_velocity": (_i32, [_vp, _vp, _vp, _vp]),
     "ktm_vgrad_eval": (_i32, [_vp, _vp, C.POINTER(C.c_double), _vp, _vp, _vp]),
+    "ktm_vgrad_pairs": (_i64, [_vp]),
     "ktm_selftest_traveltime": (_i32, [_vp, _i64, _f32, _i32, _vp, _vp, _vp, _vp, _vp]),
 }
 

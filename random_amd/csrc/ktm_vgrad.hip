@@ -64,6 +64,7 @@ struct ktm_vgrad {
     int64_t row;
     const ktm_trace_geom *geom;
     int64_t ntr;
+    int64_t pairs;         /* (trace, column) pairs passing the aperture gate */
     int bound;
 };
 
@@ -407,6 +408,7 @@ extern "C" int ktm_vgrad_bind(ktm_vgrad *p, const float *prepared, int64_t row,
             ++n;
         }
         qsort(keys, (size_t)n, sizeof(vg_key), vg_key_cmp);
+        int64_t pairs = 0;
         for (int64_t k = 0; k < n; ++k) order[k] = keys[k].idx;
         for (int64_t k = 0, h = 0; h <= p->nh; ++h) {
             while (k < n && keys[k].off < h) ++k;
@@ -424,6 +426,12 @@ extern "C" int ktm_vgrad_bind(ktm_vgrad *p, const float *prepared, int64_t row,
                 const int64_t hi = vg_bound(keys, lo, cstart[h + 1], IX + reach, 1);
                 ranges[((int64_t)X * p->nh + h) * 2] = (int32_t)lo;
                 ranges[((int64_t)X * p->nh + h) * 2 + 1] = (int32_t)hi;
+                /* the kernel's gate, same float arithmetic (no contraction) */
+                for (int64_t k = lo; k < hi; ++k) {
+                    const ktm_trace_geom *g = hg + keys[k].idx;
+                    const float cx = IX - g->cmpx, cy = p->tb.coly[0] - g->cmpy;
+                    pairs += cx * cx + cy * cy < p->tb.ap_x2thr;
+                }
             }
         }
         if (n > p->order_cap) {
@@ -447,6 +455,7 @@ extern "C" int ktm_vgrad_bind(ktm_vgrad *p, const float *prepared, int64_t row,
         p->row = row;
         p->geom = geom;
         p->ntr = ntr;
+        p->pairs = pairs;
         p->bound = 1;
     }
 done:
@@ -509,4 +518,9 @@ extern "C" int ktm_vgrad_eval(ktm_vgrad *p, const float *V, double *J, double *d
     for (size_t k = 0; k < nparts; ++k) s += p->h_part[k];
     *J = s;
     return KTM_OK;
+}
+
+extern "C" int64_t ktm_vgrad_pairs(const ktm_vgrad *p)
+{
+    return p && p->bound ? p->pairs : -1;
 }

@@ -55,6 +55,8 @@ class VelocityGradient:
         check(lib().ktm_vgrad_bind(self._h, self.prepared.data_ptr(), plan.row,
                                    self.geom.data_ptr(), traces.shape[0], stream), "ktm_vgrad_bind")
         self.cig_shape = (plan.nxp, plan.nh, plan.ns)
+        #: (trace, column) pairs in aperture; a pass evaluates pairs * ns samples
+        self.pairs = int(lib().ktm_vgrad_pairs(self._h))
 
     def close(self) -> None:
         if getattr(self, "_h", None):
