@@ -158,57 +158,138 @@ __device__ __forceinline__ float ibm_word(uint32_t w)
 {
     const float frac = (float)(w & 0x00ffffffu) * (1.0f / 16777216.0f);
     const int e4 = 4 * (int)((w >> 24) & 0x7fu) - 256;
-    uint32_t pb;
-    if (e4 > 127) pb = 0x7f800000u;
-    else if (e4 >= -126) pb = (uint32_t)(e4 + 127) << 23;
-    else if (e4 >= -149) pb = 1u << (e4 + 149);
-    else pb = 0u;
+    /* all three forms computed, then selected (no divergent branch) */
+    const uint32_t norm = (uint32_t)(e4 + 127) << 23;                 /* -126 <= e4 <= 127 */
+    const int sh = e4 + 149;                                          /* -149 <= e4 < -126 */
+    const uint32_t sub = sh >= 0 ? (1u << (sh & 31)) : 0u;
+    const uint32_t pb = e4 > 127 ? 0x7f800000u : (e4 >= -126 ? norm : sub);
     const float sf = (w >> 31) ? -frac : frac;
     return sf * __uint_as_float(pb);
+}
+
+/* Raw 32-bit word of sample t of trace i (no decode yet), and its decode:
+ * split so that a tile's loads can all be in flight before any decode. */
+template <int SRC>
+__device__ __forceinline__ uint32_t load_word(const TraceSrc &s, int64_t i, int t)
+{
+    if (SRC == SRC_F32) return reinterpret_cast<const uint32_t *>(s.p)[i * s.rec_words + t];
+    return reinterpret_cast<const uint32_t *>(s.p)[i * s.rec_words + 60 + t];
+}
+
+template <int SRC>
+__device__ __forceinline__ float decode_word(uint32_t u)
+{
+    if (SRC == SRC_F32) return __uint_as_float(u);
+    const uint32_t w = __builtin_bswap32(u);
+    return SRC == SRC_SEGY_IBM ? ibm_word(w) : __uint_as_float(w);
 }
 
 template <int SRC>
 __device__ __forceinline__ float load_sample(const TraceSrc &s, int64_t i, int t)
 {
-    if (SRC == SRC_F32) return reinterpret_cast<const float *>(s.p)[i * s.rec_words + t];
-    const uint32_t w =
-        __builtin_bswap32(reinterpret_cast<const uint32_t *>(s.p)[i * s.rec_words + 60 + t]);
-    return SRC == SRC_SEGY_IBM ? ibm_word(w) : __uint_as_float(w);
+    return decode_word<SRC>(load_word<SRC>(s, i, t));
 }
 
 /* REFERENCE mode: one lane per trace runs the two FP32 running sums of
  * Kirchhoff.c:1314-1324 in a single pass, keeping the last five values of
  * the double integral p[] in registers, and emits
  *   F[T] = (float)(-2.0*p[T] + p[T-2] + p[T+2])         (Kirchhoff.c:1392)
- * as soon as p[T+2] exists.  p[-1] = p[1] and p[-2] = p[2] are the mirror pad
- * of Kirchhoff.c:1327-1328.  No scratch buffer is needed. */
+ * as soon as p[T+2] exists (F[t-2] at step t).  p[-1] = p[1] and p[-2] =
+ * p[2] are the mirror pad of Kirchhoff.c:1327-1328.  A workgroup is one
+ * wave of 64 traces; samples pass through LDS in tiles of 64 so that every
+ * global load and store is one contiguous 256-byte run of a trace row (a
+ * lane walking its own row would touch 64 rows per instruction). */
+#define PREP_TS 32
 template <int SRC>
-__global__ void k_prep_ref(const TraceSrc src, int64_t ntr, int ns, int64_t row,
-                           float dt, const float *__restrict__ t2, float *__restrict__ out)
+__global__ void __launch_bounds__(KTM_WAVE)
+k_prep_ref(const TraceSrc src, int64_t ntr, int ns, int64_t row, float dt,
+           const float *__restrict__ t2, float *__restrict__ out)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ntr) return;
-    float *F = out + i * row;
+    /* s_inc[j][k]: the increment of integral 1 at t = t0 + k of trace j,
+     * (double)tr[t] / (4 t2[t]) (Kirchhoff.c:1314); at t = 0 the sample
+     * itself.  It does not depend on the running sum, so it is computed in
+     * the load phase, lane-parallel, and the sequential recurrence is left
+     * with one double add and two conversions per step.  Loads and stores
+     * move two traces x 32 samples per instruction (128-byte runs). */
+    __shared__ double s_inc[KTM_WAVE][PREP_TS + 1];
+    __shared__ float s_out[KTM_WAVE][PREP_TS + 1];
+    const int lane = threadIdx.x;
+    const int sub = lane & (PREP_TS - 1), half = lane / PREP_TS;
+    const int64_t tr0 = (int64_t)blockIdx.x * KTM_WAVE;
+    const int ntl = (int)min((int64_t)KTM_WAVE, ntr - tr0);
+    const int64_t i = tr0 + (lane < ntl ? lane : 0);
     const float last = load_sample<SRC>(src, i, ns - 1);
-    float q = load_sample<SRC>(src, i, 0);   /* integral 1 at t */
-    float p = q;            /* integral 2 at t */
+    float q = 0.0f, p = 0.0f;
     float pm1 = 0.0f, pm2 = 0.0f, pm3 = 0.0f, pm4 = 0.0f;
-    for (int t = 1; t <= ns + 1; ++t) {
-        if (t < ns)
-            q = (float)((double)q + (double)load_sample<SRC>(src, i, t) / (4.0 * (double)t2[t]));
-        else
-            q = q + last / dt / dt / (float)t / (float)t;
-        pm4 = pm3; pm3 = pm2; pm2 = pm1; pm1 = p;
-        p = p + q;
-        /* window: pm4..p = p[t-4]..p[t] */
-        if (t == 2)
-            F[0] = (float)(-2.0 * (double)pm2 + (double)p + (double)p);
-        else if (t == 3)
-            F[1] = (float)(-2.0 * (double)pm2 + (double)pm2 + (double)p);
-        else if (t >= 4)
-            F[t - 2] = (float)(-2.0 * (double)pm2 + (double)pm4 + (double)p);
+    for (int t0 = 0; t0 < ns + 2; t0 += PREP_TS) {
+        const int t = t0 + sub;
+        /* unconditional loads at clamped (in-bounds) positions, so that all
+         * 32 are in flight before the first wait; out-of-range values are
+         * replaced afterwards */
+        const int tc = t < ns ? t : ns - 1;
+        const double den = t < ns && t > 0 ? 4.0 * (double)t2[tc] : 1.0;
+        uint32_t wv[KTM_WAVE / 2];
+#pragma unroll
+        for (int m = 0; m < KTM_WAVE / 2; ++m)
+            wv[m] = load_word<SRC>(src, tr0 + min(2 * m + half, ntl - 1), tc);
+        float xv[KTM_WAVE / 2];
+#pragma unroll
+        for (int m = 0; m < KTM_WAVE / 2; ++m)
+            xv[m] = t < ns ? decode_word<SRC>(wv[m]) : 0.0f;
+#pragma unroll
+        for (int m = 0; m < KTM_WAVE / 2; ++m)
+            s_inc[2 * m + half][sub] = t == 0 ? (double)xv[m] : (double)xv[m] / den;
+        __syncthreads();
+        const int s_end = min(PREP_TS, ns + 2 - t0);
+        if (t0 >= 4 && t0 + PREP_TS <= ns) {
+            /* interior tile: every step is the t >= 4, t < ns case; unrolled
+             * so that the LDS reads and the F arithmetic overlap the
+             * recurrence (one double add and two conversions per step) */
+            double inc[PREP_TS];
+#pragma unroll
+            for (int k = 0; k < PREP_TS; ++k) inc[k] = s_inc[lane][k];
+#pragma unroll
+            for (int k = 0; k < PREP_TS; ++k) {
+                q = (float)((double)q + inc[k]);
+                pm4 = pm3; pm3 = pm2; pm2 = pm1; pm1 = p;
+                p = p + q;
+                s_out[lane][k] = (float)(-2.0 * (double)pm2 + (double)pm4 + (double)p);
+            }
+        } else for (int k = 0; k < s_end; ++k) {
+            const int tk = t0 + k;
+            const double inc = s_inc[lane][k];
+            if (tk == 0) {
+                q = (float)inc;   /* integral 1 at t */
+                p = q;            /* integral 2 at t */
+                continue;
+            }
+            if (tk < ns)
+                q = (float)((double)q + inc);
+            else
+                q = q + last / dt / dt / (float)tk / (float)tk;
+            pm4 = pm3; pm3 = pm2; pm2 = pm1; pm1 = p;
+            p = p + q;
+            /* window: pm4..p = p[t-4]..p[t]; F[t-2] goes to s_out[.][k] */
+            if (tk == 2)
+                s_out[lane][k] = (float)(-2.0 * (double)pm2 + (double)p + (double)p);
+            else if (tk == 3)
+                s_out[lane][k] = (float)(-2.0 * (double)pm2 + (double)pm2 + (double)p);
+            else if (tk >= 4)
+                s_out[lane][k] = (float)(-2.0 * (double)pm2 + (double)pm4 + (double)p);
+        }
+        __syncthreads();
+        const int T = t0 - 2 + sub;           /* F index held in s_out[.][sub] */
+        if (sub < s_end && T >= 0 && T < ns) {
+#pragma unroll 8
+            for (int m = 0; m < KTM_WAVE / 2; ++m) {
+                const int j = 2 * m + half;
+                if (j < ntl) out[(tr0 + j) * row + T] = s_out[j][sub];
+            }
+        }
+        __syncthreads();
     }
-    for (int64_t t = ns; t < row; ++t) F[t] = 0.0f;
+    for (int j = 0; j < ntl; ++j)
+        for (int64_t T = ns + lane; T < row; T += KTM_WAVE) out[(tr0 + j) * row + T] = 0.0f;
 }
 
 /* CLOSED mode: F as the 3-tap filter of x[t] = trace[min(t,ns-1)] * w[t].
@@ -294,31 +375,80 @@ __device__ __forceinline__ float f16_to_f32(uint16_t u)
     return (float)h;
 }
 
-/* One lane per trace: scale = max |tr| (1 for an all-zero trace); the trace
- * divided by scale is rounded to BF16 or FP16 with first-order error
- * feedback (the rounding error of sample t is carried into sample t+1), which
- * moves the quantisation noise towards the Nyquist frequency, where the
- * [1,2,1] anti-alias taps of the migration have their zero. */
+/* scale = max |tr| (1 for an all-zero trace); the trace divided by scale is
+ * rounded to BF16 or FP16 with first-order error feedback (the rounding error
+ * of sample t is carried into sample t+1), which moves the quantisation noise
+ * towards the Nyquist frequency, where the [1,2,1] anti-alias taps of the
+ * migration have their zero.  A workgroup is one wave of 64 traces: the
+ * max-abs sweep reads two traces x 32 samples per load (max is exact in any
+ * order); the sequential error-feedback sweep stages 32-sample tiles through
+ * LDS like k_prep_ref, one lane per trace. */
 template <int DT, int SRC>
-__global__ void k_quantize(const TraceSrc src, int64_t ntr, int ns, int64_t row,
-                           uint16_t *__restrict__ q, ktm_trace_geom *__restrict__ geom)
+__global__ void __launch_bounds__(KTM_WAVE)
+k_quantize(const TraceSrc src, int64_t ntr, int ns, int64_t row, uint16_t *__restrict__ q,
+           ktm_trace_geom *__restrict__ geom)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ntr) return;
-    float m = 0.0f;
-    for (int t = 0; t < ns; ++t) m = fmaxf(m, fabsf(load_sample<SRC>(src, i, t)));
-    const float scale = m > 0.0f ? m : 1.0f;
-    const float inv = 1.0f / scale;
-    uint16_t *o = q + i * row;
-    float e = 0.0f;
-    for (int t = 0; t < ns; ++t) {
-        const float v = load_sample<SRC>(src, i, t) * inv + e;
-        const uint16_t h = DT == KTM_TRACE_BF16 ? f32_to_bf16(v) : f32_to_f16(v);
-        o[t] = h;
-        e = v - (DT == KTM_TRACE_BF16 ? bf16_to_f32(h) : f16_to_f32(h));
+    __shared__ float s_in[KTM_WAVE][PREP_TS + 1];
+    __shared__ uint16_t s_q[KTM_WAVE][PREP_TS + 2];
+    __shared__ float s_scale[KTM_WAVE];
+    const int lane = threadIdx.x;
+    const int sub = lane & (PREP_TS - 1), half = lane / PREP_TS;
+    const int64_t tr0 = (int64_t)blockIdx.x * KTM_WAVE;
+    const int ntl = (int)min((int64_t)KTM_WAVE, ntr - tr0);
+    for (int m = 0; m < KTM_WAVE / 2; ++m) {
+        const int j = min(2 * m + half, ntl - 1);
+        float mx = 0.0f;
+#pragma unroll 8
+        for (int t = sub; t < ns; t += PREP_TS) mx = fmaxf(mx, fabsf(load_sample<SRC>(src, tr0 + j, t)));
+        for (int o = PREP_TS / 2; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+        if (sub == 0) s_scale[2 * m + half] = mx;
     }
-    for (int64_t t = ns; t < row; ++t) o[t] = 0;
-    geom[i].scale = scale;
+    __syncthreads();
+    const float m_own = s_scale[lane];
+    const float scale = m_own > 0.0f ? m_own : 1.0f;
+    const float inv = 1.0f / scale;
+    float e = 0.0f;
+    for (int t0 = 0; t0 < ns; t0 += PREP_TS) {
+        const int t = t0 + sub;
+        const int tc = t < ns ? t : ns - 1;
+        uint32_t wv[KTM_WAVE / 2];
+#pragma unroll
+        for (int m = 0; m < KTM_WAVE / 2; ++m)
+            wv[m] = load_word<SRC>(src, tr0 + min(2 * m + half, ntl - 1), tc);
+#pragma unroll
+        for (int m = 0; m < KTM_WAVE / 2; ++m) s_in[2 * m + half][sub] = decode_word<SRC>(wv[m]);
+        __syncthreads();
+        const int s_end = min(PREP_TS, ns - t0);
+        if (s_end == PREP_TS) {
+            float xk[PREP_TS];
+#pragma unroll
+            for (int k = 0; k < PREP_TS; ++k) xk[k] = s_in[lane][k];
+#pragma unroll
+            for (int k = 0; k < PREP_TS; ++k) {
+                const float v = xk[k] * inv + e;
+                const uint16_t h = DT == KTM_TRACE_BF16 ? f32_to_bf16(v) : f32_to_f16(v);
+                s_q[lane][k] = h;
+                e = v - (DT == KTM_TRACE_BF16 ? bf16_to_f32(h) : f16_to_f32(h));
+            }
+        } else for (int k = 0; k < s_end; ++k) {
+            const float v = s_in[lane][k] * inv + e;
+            const uint16_t h = DT == KTM_TRACE_BF16 ? f32_to_bf16(v) : f32_to_f16(v);
+            s_q[lane][k] = h;
+            e = v - (DT == KTM_TRACE_BF16 ? bf16_to_f32(h) : f16_to_f32(h));
+        }
+        __syncthreads();
+        if (sub < s_end) {
+#pragma unroll 8
+            for (int m = 0; m < KTM_WAVE / 2; ++m) {
+                const int j = 2 * m + half;
+                if (j < ntl) q[(tr0 + j) * row + t] = s_q[j][sub];
+            }
+        }
+        __syncthreads();
+    }
+    for (int j = 0; j < ntl; ++j)
+        for (int64_t T = ns + lane; T < row; T += KTM_WAVE) q[(tr0 + j) * row + T] = 0;
+    if (lane < ntl) geom[tr0 + lane].scale = scale;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1135,9 +1265,9 @@ static int prep_launch(const ktm_plan *p, const TraceSrc src, int64_t ntr, int p
     const int ns = p->tb.s.ns;
     if (dtype == KTM_TRACE_F32) {
         if (prep_mode == KTM_PREP_REFERENCE) {
-            const int nt = 64;
-            hipLaunchKernelGGL(k_prep_ref<SRC>, dim3((unsigned)((ntr + nt - 1) / nt)), dim3(nt), 0,
-                               s, src, ntr, ns, p->row, p->tb.s.dt, p->d_t2, (float *)prepared);
+            hipLaunchKernelGGL(k_prep_ref<SRC>, dim3((unsigned)((ntr + KTM_WAVE - 1) / KTM_WAVE)),
+                               dim3(KTM_WAVE), 0, s, src, ntr, ns, p->row, p->tb.s.dt, p->d_t2,
+                               (float *)prepared);
         } else {
             if (ntr > 65535 * 1024LL) return KTM_ERR_UNSUPPORTED;
             const int nt = 256;
@@ -1152,7 +1282,7 @@ static int prep_launch(const ktm_plan *p, const TraceSrc src, int64_t ntr, int p
             }
         }
     } else {
-        const int nt = 64;
+        const int nt = KTM_WAVE;
         const dim3 grid((unsigned)((ntr + nt - 1) / nt));
         if (dtype == KTM_TRACE_BF16)
             hipLaunchKernelGGL((k_quantize<KTM_TRACE_BF16, SRC>), grid, dim3(nt), 0, s, src, ntr, ns,
@@ -1213,19 +1343,20 @@ extern "C" int ktm_ingest(const ktm_plan *p, const void *raw, int64_t ntr, int f
     return prep_launch<SRC_SEGY_IEEE>(p, src, ntr, prep_mode, dtype, prepared, geom, s);
 }
 
-/* raw SEG-Y records -> FP32 traces and header words, one lane per sample */
+/* raw SEG-Y records -> FP32 traces and header words: one workgroup per
+ * record (the sample loop is contiguous, no index division) */
 template <int SRC>
 __global__ void k_segy_decode(const TraceSrc src, int64_t ntr, int ns, float *__restrict__ traces,
                               int32_t *__restrict__ coords)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ntr * (int64_t)ns) return;
-    const int64_t i = e / ns;
-    const int t = (int)(e - i * ns);
-    if (traces) traces[e] = load_sample<SRC>(src, i, t);
-    if (coords && t < 4)
-        coords[4 * i + t] = (int32_t)__builtin_bswap32(
-            reinterpret_cast<const uint32_t *>(src.p)[i * src.rec_words + 18 + t]);
+    const int64_t i = blockIdx.x;
+    if (i >= ntr) return;
+    if (traces)
+        for (int t = threadIdx.x; t < ns; t += blockDim.x)
+            traces[i * ns + t] = load_sample<SRC>(src, i, t);
+    if (coords && threadIdx.x < 4)
+        coords[4 * i + threadIdx.x] = (int32_t)__builtin_bswap32(
+            reinterpret_cast<const uint32_t *>(src.p)[i * src.rec_words + 18 + threadIdx.x]);
 }
 
 extern "C" int ktm_segy_decode(const void *raw, int64_t ntr, int ns, int format, float *traces,
@@ -1236,10 +1367,9 @@ extern "C" int ktm_segy_decode(const void *raw, int64_t ntr, int ns, int format,
     if (!raw || ((uintptr_t)raw & 3u)) return KTM_ERR_ARG;
     if (format != KTM_SEGY_IBM && format != KTM_SEGY_IEEE) return KTM_ERR_ARG;
     const TraceSrc src = {raw, 60 + (int64_t)ns};
-    const int64_t n = ntr * (int64_t)ns;
     const int nt = 256;
-    const dim3 grid((unsigned)((n + nt - 1) / nt));
-    if ((n + nt - 1) / nt > 0x7fffffffLL) return KTM_ERR_UNSUPPORTED;
+    if (ntr > 0x7fffffffLL) return KTM_ERR_UNSUPPORTED;
+    const dim3 grid((unsigned)ntr);
     hipStream_t s = (hipStream_t)stream;
     if (format == KTM_SEGY_IBM)
         hipLaunchKernelGGL(k_segy_decode<SRC_SEGY_IBM>, grid, dim3(nt), 0, s, src, ntr, ns, traces,
@@ -1279,18 +1409,19 @@ __device__ __forceinline__ uint32_t float_to_ibm(float v)
     return (b & 0x80000000u) | ((uint32_t)(e16 + 64) << 24) | mant;
 }
 
-/* samples of ntr records: records[r][60 + t] = big-endian word of traces[r][t] */
+/* samples of ntr records: records[r][60 + t] = big-endian word of
+ * traces[r][t]; one workgroup per record */
 template <int FMT>
 __global__ void k_encode_samples(const float *__restrict__ traces, int64_t ntr, int ns,
                                  uint32_t *__restrict__ rec)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ntr * (int64_t)ns) return;
-    const int64_t r = e / ns;
-    const int t = (int)(e - r * ns);
-    const float v = traces[e];
-    const uint32_t w = FMT == KTM_SEGY_IBM ? float_to_ibm(v) : __float_as_uint(v);
-    rec[r * (60 + (int64_t)ns) + 60 + t] = __builtin_bswap32(w);
+    const int64_t r = blockIdx.x;
+    if (r >= ntr) return;
+    for (int t = threadIdx.x; t < ns; t += blockDim.x) {
+        const float v = traces[r * ns + t];
+        const uint32_t w = FMT == KTM_SEGY_IBM ? float_to_ibm(v) : __float_as_uint(v);
+        rec[r * (60 + (int64_t)ns) + 60 + t] = __builtin_bswap32(w);
+    }
 }
 
 /* headers [ntr][60 words] into the records (or zeros) */
@@ -1379,17 +1510,15 @@ extern "C" int ktm_segy_encode(const float *traces, int64_t ntr, int ns, int for
     if (format != KTM_SEGY_IBM && format != KTM_SEGY_IEEE) return KTM_ERR_ARG;
     hipStream_t s = (hipStream_t)stream;
     const int nt = 256;
-    const int64_t n = ntr * (int64_t)ns;
-    if ((n + nt - 1) / nt > 0x7fffffffLL || (ntr * 60 + nt - 1) / nt > 0x7fffffffLL)
-        return KTM_ERR_UNSUPPORTED;
+    if (ntr > 0x7fffffffLL || (ntr * 60 + nt - 1) / nt > 0x7fffffffLL) return KTM_ERR_UNSUPPORTED;
     hipLaunchKernelGGL(k_copy_headers, dim3((unsigned)((ntr * 60 + nt - 1) / nt)), dim3(nt), 0, s,
                        (const uint32_t *)headers, ntr, ns, (uint32_t *)records);
     if (format == KTM_SEGY_IBM)
-        hipLaunchKernelGGL(k_encode_samples<KTM_SEGY_IBM>, dim3((unsigned)((n + nt - 1) / nt)),
-                           dim3(nt), 0, s, traces, ntr, ns, (uint32_t *)records);
+        hipLaunchKernelGGL(k_encode_samples<KTM_SEGY_IBM>, dim3((unsigned)ntr), dim3(nt), 0, s,
+                           traces, ntr, ns, (uint32_t *)records);
     else
-        hipLaunchKernelGGL(k_encode_samples<KTM_SEGY_IEEE>, dim3((unsigned)((n + nt - 1) / nt)),
-                           dim3(nt), 0, s, traces, ntr, ns, (uint32_t *)records);
+        hipLaunchKernelGGL(k_encode_samples<KTM_SEGY_IEEE>, dim3((unsigned)ntr), dim3(nt), 0, s,
+                           traces, ntr, ns, (uint32_t *)records);
     HIP_TRY(hipGetLastError());
     return KTM_OK;
 }

@@ -80,3 +80,45 @@ def test_reference_fp32_error_is_the_references_own(cuda, area):
     late = rel_l2(km[..., 1000:], km64[..., 1000:])
     print(f"case: FP32 reference vs FP64: relL2 t<300 {early:.3e}, t>=1000 {late:.3e}", flush=True)
     assert early < late
+
+
+def _bf16_rne(v32: np.ndarray) -> np.ndarray:
+    """float32 -> bf16 bits, round to nearest even (finite inputs)."""
+    u = v32.view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+@pytest.mark.parametrize("dtype", [TRACE_BF16, TRACE_F16], ids=["bf16", "fp16"])
+def test_quantized_storage_matches_numpy_error_feedback(cuda, dtype):
+    """k_quantize against a numpy restatement: scale = max|tr|, v = tr/scale
+    (times the float reciprocal) + carried error, RNE to 16 bits, error =
+    v - decoded; 130 traces (a partial workgroup) of 777 samples."""
+    rng = np.random.default_rng(21)
+    ntr, ns = 130, 777
+    tr = (rng.standard_normal((ntr, ns)) * rng.uniform(1e-3, 1e3, (ntr, 1))).astype(np.float32)
+    tr[5] = 0.0                                          # scale 1
+    sv = surveys.line2d(nshots=1, ncols=16, ns=ns, nscat=1)
+    plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
+    coords = np.repeat(sv.coords[:1], ntr, axis=0)
+    prep, geom = plan.prep(torch.from_numpy(tr).to(cuda), torch.from_numpy(coords).to(cuda),
+                           PREP_CLOSED, dtype)
+    got = prep.cpu().numpy().view(np.uint16)
+    scale_gpu = geom.cpu().numpy()[:, 7]
+    m = np.abs(tr).max(axis=1)
+    scale = np.where(m > 0, m, np.float32(1.0)).astype(np.float32)
+    assert np.array_equal(scale_gpu.view(np.uint32), scale.view(np.uint32))
+    inv = (np.float32(1.0) / scale).astype(np.float32)
+    e = np.zeros(ntr, np.float32)
+    want = np.zeros((ntr, plan.row), np.uint16)
+    for t in range(ns):
+        v = (tr[:, t] * inv + e).astype(np.float32)
+        if dtype == TRACE_BF16:
+            h = _bf16_rne(v)
+            dec = (h.astype(np.uint32) << 16).view(np.float32)
+        else:
+            h16 = v.astype(np.float16)
+            h = h16.view(np.uint16)
+            dec = h16.astype(np.float32)
+        want[:, t] = h
+        e = (v - dec).astype(np.float32)
+    assert np.array_equal(got, want)
