@@ -535,19 +535,23 @@ __device__ __forceinline__ float fast_q(float dIS2, float dIR2, float r, float t
     return d * rdt;
 }
 
-/* An integer within KTM_FAST_EPS*(q+1) of q: floor(q) may differ from the
- * reference's floor(Q).  Beyond ns+1 both are >= ns (sample discarded). */
-__device__ __forceinline__ bool fast_ambiguous(float q, float nsf1)
+/* q clamped to nsh = ns + 0.5.  q >= 0 (a sum of square roots).  Beyond
+ * ns + 0.5 the reference's Q is > ns as well (|q - Q| <= 1e-6 q), so the
+ * sample is discarded either way; the clamped value truncates to ns and is
+ * half a sample from an integer, hence never ambiguous.  One clamp serves
+ * the index and the ambiguity test. */
+__device__ __forceinline__ float fast_clamp(float q, float nsh) { return fminf(q, nsh); }
+
+/* An integer within KTM_FAST_EPS*(q+1) of the clamped q: floor(q) may
+ * differ from the reference's floor(Q). */
+__device__ __forceinline__ bool fast_ambiguous(float qc)
 {
-    const float dev = fabsf(q - __builtin_rintf(q));
-    return q < nsf1 && dev <= __builtin_fmaf(q, KTM_FAST_EPS, KTM_FAST_EPS);
+    const float dev = fabsf(qc - __builtin_rintf(qc));
+    return dev <= __builtin_fmaf(qc, KTM_FAST_EPS, KTM_FAST_EPS);
 }
 
-/* floor(q) clamped to [0, ns] (q >= 0) */
-__device__ __forceinline__ int fast_T(float q, float nsf)
-{
-    return (int)__builtin_amdgcn_fmed3f(q, 0.0f, nsf);
-}
+/* floor of the clamped q, in [0, ns] */
+__device__ __forceinline__ int fast_T(float qc) { return (int)qc; }
 
 /* One workgroup = 8 waves = NCOL image columns x W time segments.  Each lane
  * owns K samples t = seg*64K + 64k + lane of its column: the image values
@@ -592,7 +596,7 @@ k_migrate(const MigArgs a)
     const float scx = a.slab_cx[job], scy = a.slab_cy[job];
     const float sthr = a.slab_x2thr[job];
     const int ns = a.ns;
-    const float nsf = (float)ns, nsf1 = (float)(ns + 1);
+    const float nsh = (float)ns + 0.5f;   /* clamp of q */
 
     /* conservative tile test: the CMP must be within aperture of some
      * column of the tile (exact per-column gates follow) */
@@ -705,9 +709,9 @@ k_migrate(const MigArgs a)
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
                         const float t2k = t2v[k];
-                        const float q = fast_q(dIS2, dIR2, r[k], t2k, a.rdt);
-                        int T = fast_T(q, nsf);
-                        if (fast_ambiguous(q, nsf1))
+                        const float qc = fast_clamp(fast_q(dIS2, dIR2, r[k], t2k, a.rdt), nsh);
+                        int T = fast_T(qc);
+                        if (fast_ambiguous(qc))
                             T = exact_T(dIS2, dIR2, t2k,
                                         sV2c[seg * K * KTM_WAVE + k * KTM_WAVE + lane], r[k], a.dt,
                                         a.rdt, ns);
@@ -932,9 +936,9 @@ __global__ void k_selftest_traveltime(const float4 *__restrict__ in, int64_t n, 
     const float4 v = in[i];
     const float r = 1.0f / v.w, rdt = 1.0f / dt;
     const float q = fast_q(v.x, v.y, r, v.z, rdt);
-    const float nsf = (float)ns, nsf1 = (float)(ns + 1);
-    int T = fast_T(q, nsf);
-    if (fast_ambiguous(q, nsf1)) T = exact_T(v.x, v.y, v.z, v.w, r, dt, rdt, ns);
+    const float nsf = (float)ns, qc = fast_clamp(q, (float)ns + 0.5f);
+    int T = fast_T(qc);
+    if (fast_ambiguous(qc)) T = exact_T(v.x, v.y, v.z, v.w, r, dt, rdt, ns);
     /* the reference computed independently of exact_T: the compiler's
      * correctly rounded division and double square root */
     const float s1 = v.z + v.x / v.w, s2 = v.z + v.y / v.w;
@@ -1019,7 +1023,7 @@ __global__ void __launch_bounds__(8 * KTM_WAVE) k_demigrate(const DemigArgs a)
     for (int t = tid; t < ns + 4; t += 8 * KTM_WAVE) sg[t] = 0.0f;
     __syncthreads();
     const ktm_trace_geom g = a.geom[i];
-    const float nsf = (float)ns, nsf1 = (float)(ns + 1);
+    const float nsh = (float)ns + 0.5f;   /* clamp of q */
     if (g.off >= 0 && g.off < a.nh) {
         /* columns whose centre can be within the aperture of the CMP */
         int xlo = 0, xhi = a.nxp - 1, ylo = 0, yhi = a.nyp - 1;
@@ -1058,9 +1062,9 @@ __global__ void __launch_bounds__(8 * KTM_WAVE) k_demigrate(const DemigArgs a)
                 const int t = seg * K * KTM_WAVE + k * KTM_WAVE + lane;
                 if (t < ns) {
                     const float r = a.pr[pixc + t], t2 = a.t2[t];
-                    const float q = fast_q(dIS2, dIR2, r, t2, a.rdt);
-                    int T = fast_T(q, nsf);
-                    if (fast_ambiguous(q, nsf1))
+                    const float qc = fast_clamp(fast_q(dIS2, dIR2, r, t2, a.rdt), nsh);
+                    int T = fast_T(qc);
+                    if (fast_ambiguous(qc))
                         T = exact_T(dIS2, dIR2, t2, a.pv2[pixc + t], r, a.dt, a.rdt, ns);
                     if (T < ns) atomicAdd(&sg[T], mk[t]);
                 }
