@@ -535,11 +535,12 @@ __device__ __forceinline__ float fast_q(float dIS2, float dIR2, float r, float t
     return d * rdt;
 }
 
-/* q clamped to nsh = ns + 0.5.  q >= 0 (a sum of square roots).  Beyond
- * ns + 0.5 the reference's Q is > ns as well (|q - Q| <= 1e-6 q), so the
- * sample is discarded either way; the clamped value truncates to ns and is
- * half a sample from an integer, hence never ambiguous.  One clamp serves
- * the index and the ambiguity test. */
+/* q clamped to nsh = ns + 0.5 (demigration).  q >= 0 (a sum of square
+ * roots).  Beyond ns + 0.5 the reference's Q is > ns as well (|q - Q| <=
+ * 1e-6 q), so the sample is discarded either way; the clamped value
+ * truncates to ns and is half a sample from an integer, hence never
+ * ambiguous.  One clamp serves the index and the ambiguity test.  (The
+ * migration kernel clamps at ns instead, see there.) */
 __device__ __forceinline__ float fast_clamp(float q, float nsh) { return fminf(q, nsh); }
 
 /* An integer within KTM_FAST_EPS*(q+1) of the clamped q: floor(q) may
@@ -596,7 +597,7 @@ k_migrate(const MigArgs a)
     const float scx = a.slab_cx[job], scy = a.slab_cy[job];
     const float sthr = a.slab_x2thr[job];
     const int ns = a.ns;
-    const float nsh = (float)ns + 0.5f;   /* clamp of q */
+    const float nsf = (float)ns;
 
     /* conservative tile test: the CMP must be within aperture of some
      * column of the tile (exact per-column gates follow) */
@@ -615,7 +616,7 @@ k_migrate(const MigArgs a)
     const int64_t pix = (((int64_t)Xc * a.nyp + Yc) * a.nh + h) * ns;
     const int64_t vbase = ((int64_t)a.velx[Xc] * a.nvel_xl + a.vely[Yc]) * a.nsm;
     float r[K], t2v[K], acc[K], cmp[K];
-    int cnt[K];
+    int cnt[K];                             /* passing traces with T >= ns, per slot */
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int tl = seg * K * KTM_WAVE + k * KTM_WAVE + lane;
@@ -630,8 +631,10 @@ k_migrate(const MigArgs a)
             if (KAHAN && col_ok) cmp[k] = a.comp[pix + tl];
         } else {
             sV2c[tl] = 1.0f;
+            /* padding lane: q = 2 sqrt(dt^2/16) / dt = 0.5, never flagged by the
+             * ambiguity test; it reads F[0] into a value that is not stored */
             r[k] = 0.0f;
-            t2v[k] = 3.0e38f;   /* q becomes huge: T = ns, nothing added */
+            t2v[k] = 0.0625f * a.dt * a.dt;
             acc[k] = 0.0f;
         }
     }
@@ -709,12 +712,23 @@ k_migrate(const MigArgs a)
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
                         const float t2k = t2v[k];
-                        const float qc = fast_clamp(fast_q(dIS2, dIR2, r[k], t2k, a.rdt), nsh);
+                        const float q = fast_q(dIS2, dIR2, r[k], t2k, a.rdt);
+                        /* clamped at ns: a sample at or past the end of the
+                         * trace lands on an integer and takes the rare path
+                         * with the ambiguous ones, where the samples that
+                         * do not count towards fold are counted, so the
+                         * common path has no compare for fold */
+                        const float qc = fminf(q, nsf);
                         int T = fast_T(qc);
-                        if (fast_ambiguous(qc))
-                            T = exact_T(dIS2, dIR2, t2k,
-                                        sV2c[seg * K * KTM_WAVE + k * KTM_WAVE + lane], r[k], a.dt,
-                                        a.rdt, ns);
+                        if (fast_ambiguous(qc)) {
+                            if (q - nsf >= __builtin_fmaf(q, KTM_FAST_EPS, KTM_FAST_EPS))
+                                T = ns;   /* Q >= ns as well: past the trace */
+                            else
+                                T = exact_T(dIS2, dIR2, t2k,
+                                            sV2c[seg * K * KTM_WAVE + k * KTM_WAVE + lane], r[k],
+                                            a.dt, a.rdt, ns);
+                            cnt[k] += T >= ns ? 1 : 0;
+                        }
                         const float f = Fr[T];
                         if (KAHAN) {
                             const float y = f - cmp[k];
@@ -724,7 +738,6 @@ k_migrate(const MigArgs a)
                         } else {
                             acc[k] = acc[k] + f;
                         }
-                        cnt[k] += T < ns ? 1 : 0;
                     }
                 }
             }
@@ -738,7 +751,9 @@ k_migrate(const MigArgs a)
             const int t = seg * K * KTM_WAVE + k * KTM_WAVE + lane;
             if (t < ns) {
                 a.km[pix + t] = acc[k];
-                a.fold[pix + t] = a.fold[pix + t] + (float)cnt[k];
+                /* fold adds the passing traces whose sample was inside the
+                 * trace (Kirchhoff.c:1387-1396): all of them minus cnt */
+                a.fold[pix + t] = a.fold[pix + t] + (float)((int)npass - cnt[k]);
                 if (KAHAN) a.comp[pix + t] = cmp[k];
             }
         }
@@ -936,9 +951,12 @@ __global__ void k_selftest_traveltime(const float4 *__restrict__ in, int64_t n, 
     const float4 v = in[i];
     const float r = 1.0f / v.w, rdt = 1.0f / dt;
     const float q = fast_q(v.x, v.y, r, v.z, rdt);
-    const float nsf = (float)ns, qc = fast_clamp(q, (float)ns + 0.5f);
+    /* the decision procedure of k_migrate */
+    const float nsf = (float)ns, qc = fminf(q, nsf);
     int T = fast_T(qc);
-    if (fast_ambiguous(qc)) T = exact_T(v.x, v.y, v.z, v.w, r, dt, rdt, ns);
+    if (fast_ambiguous(qc))
+        T = q - nsf >= __builtin_fmaf(q, KTM_FAST_EPS, KTM_FAST_EPS)
+                ? ns : exact_T(v.x, v.y, v.z, v.w, r, dt, rdt, ns);
     /* the reference computed independently of exact_T: the compiler's
      * correctly rounded division and double square root */
     const float s1 = v.z + v.x / v.w, s2 = v.z + v.y / v.w;
