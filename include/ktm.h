@@ -342,9 +342,11 @@ float ktm_gate_threshold(double A);
 /* out_hw[i] = the hardware square root v_sqrt_f32(x[i]) used by the fast
  * traveltime path (its error bound is an input of that path's proof). */
 int ktm_selftest_sqrt(const float *x, int64_t n, float *out_hw, void *stream);
-/* in4[n][4] = (dIS2, dIR2, t2, v2): q_fast (fast path), q_ref = RN(dsr/dt)
- * with the arithmetic of Kirchhoff.c:1381-1384, and the sample index the
- * migration kernel selects (t_kernel) vs the reference's (t_ref). */
+/* in4[n][4] = (dIS2, dIR2, t, v2), t a sample index held as a float (t2
+ * is formed as in the traveltime table, Kirchhoff.c:1246-1247): q_fast (fast
+ * path), q_ref = RN(dsr/dt) with the arithmetic of Kirchhoff.c:1381-1384,
+ * and the sample index the migration kernel selects (t_kernel) vs the
+ * reference's (t_ref). */
 int ktm_selftest_traveltime(const float *in4, int64_t n, float dt, int ns, float *q_fast,
                             float *q_ref, int32_t *t_kernel, int32_t *t_ref, void *stream);
 

@@ -535,6 +535,29 @@ __device__ __forceinline__ float fast_q(float dIS2, float dIR2, float r, float t
     return d * rdt;
 }
 
+/* The migration kernel's form, in samples from the start: with
+ * rs = RN(1/(v2 dt^2)) (product and quotient in double, one rounding) and
+ * tq = 0.25 t^2 (exact for t < 4096, else one rounding),
+ *   q = RN( sqrt_hw(fma(dIS2, rs, tq)) + sqrt_hw(fma(dIR2, rs, tq)) ).
+ * a1 = RN(fma(dIS2, rs, tq)) is within 2u of (t2e + dIS2/v2)/dt^2 (t2e =
+ * t^2 dt^2 / 4), the reference's s1 = RN(t2 + RN(dIS2/v2)) within 2u of
+ * t2e + dIS2/v2 (its t2 is t2e rounded once): 4u on the radicand, 2u after
+ * the square root, plus 2u for v_sqrt_f32; the sum's rounding u; the
+ * reference's RN(dsr) and RN(dsr/dt) 2u: |q - Q| <= 7u Q to first order,
+ * inside KTM_FAST_EPS = 10u.  No multiply by 1/dt is left in the hot loop. */
+__device__ __forceinline__ float fast_qs(float dIS2, float dIR2, float rs, float tq)
+{
+    return __builtin_amdgcn_sqrtf(__builtin_fmaf(dIS2, rs, tq)) +
+           __builtin_amdgcn_sqrtf(__builtin_fmaf(dIR2, rs, tq));
+}
+
+/* per-sample constants of fast_qs */
+__device__ __forceinline__ float fast_rs(float v2, float dt)
+{
+    return (float)(1.0 / ((double)v2 * (double)dt * (double)dt));
+}
+__device__ __forceinline__ float fast_tq(int t) { return (float)(0.25 * (double)t * (double)t); }
+
 /* q clamped to nsh = ns + 0.5 (demigration).  q >= 0 (a sum of square
  * roots).  Beyond ns + 0.5 the reference's Q is > ns as well (|q - Q| <=
  * 1e-6 q), so the sample is discarded either way; the clamped value
@@ -610,12 +633,18 @@ k_migrate(const MigArgs a)
     const float reach2 = reach * reach;
 
 
-    /* v^2 of this column for the exact path lives in LDS; the fast path keeps
-     * 1/v^2 and t2 in registers */
+    /* v^2 of this column for the exact path lives in LDS (t2 is read from
+     * the table there); the fast path keeps rs = RN(1/(v^2 dt^2)), tq =
+     * t^2/4 and, for the exact path, RN(1/v^2) in registers */
     float *sV2c = sV2 + cw * (W * K * KTM_WAVE);
     const int64_t pix = (((int64_t)Xc * a.nyp + Yc) * a.nh + h) * ns;
     const int64_t vbase = ((int64_t)a.velx[Xc] * a.nvel_xl + a.vely[Yc]) * a.nsm;
-    float r[K], t2v[K], acc[K], cmp[K];
+    /* The plain variants use fast_qs (radicands pre-scaled by 1/dt^2: rs
+     * and tq in registers, t2 from the table on the exact path); the Kahan
+     * variants, with K more registers for the compensation, keep the
+     * fast_q form (1/v^2 and t2 in registers, a multiply by 1/dt). */
+    constexpr bool SCALED = !KAHAN;
+    float r[K], rs[K], tq[K], acc[K], cmp[K];
     int cnt[K];                             /* passing traces with T >= ns, per slot */
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -626,15 +655,17 @@ k_migrate(const MigArgs a)
             const float v2 = a.vel2[vbase + a.t1[tl]];
             sV2c[tl] = v2;
             r[k] = 1.0f / v2;
-            t2v[k] = a.t2[tl];
+            rs[k] = SCALED ? fast_rs(v2, a.dt) : 0.0f;
+            tq[k] = SCALED ? fast_tq(tl) : a.t2[tl];
             acc[k] = col_ok ? a.km[pix + tl] : 0.0f;
             if (KAHAN && col_ok) cmp[k] = a.comp[pix + tl];
         } else {
             sV2c[tl] = 1.0f;
-            /* padding lane: q = 2 sqrt(dt^2/16) / dt = 0.5, never flagged by the
-             * ambiguity test; it reads F[0] into a value that is not stored */
+            /* padding lane: q = 0.5, never flagged by the ambiguity test; it
+             * reads F[0] into a value that is not stored */
             r[k] = 0.0f;
-            t2v[k] = 0.0625f * a.dt * a.dt;
+            rs[k] = 0.0f;
+            tq[k] = SCALED ? 0.0625f : 0.0625f * a.dt * a.dt;
             acc[k] = 0.0f;
         }
     }
@@ -711,8 +742,8 @@ k_migrate(const MigArgs a)
                      * the reference's. */
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
-                        const float t2k = t2v[k];
-                        const float q = fast_q(dIS2, dIR2, r[k], t2k, a.rdt);
+                        const float q = SCALED ? fast_qs(dIS2, dIR2, rs[k], tq[k])
+                                               : fast_q(dIS2, dIR2, r[k], tq[k], a.rdt);
                         /* clamped at ns: a sample at or past the end of the
                          * trace lands on an integer and takes the rare path
                          * with the ambiguous ones, where the samples that
@@ -723,10 +754,11 @@ k_migrate(const MigArgs a)
                         if (fast_ambiguous(qc)) {
                             if (q - nsf >= __builtin_fmaf(q, KTM_FAST_EPS, KTM_FAST_EPS))
                                 T = ns;   /* Q >= ns as well: past the trace */
-                            else
-                                T = exact_T(dIS2, dIR2, t2k,
-                                            sV2c[seg * K * KTM_WAVE + k * KTM_WAVE + lane], r[k],
+                            else {
+                                const int tl = seg * K * KTM_WAVE + k * KTM_WAVE + lane;
+                                T = exact_T(dIS2, dIR2, SCALED ? a.t2[tl] : tq[k], sV2c[tl], r[k],
                                             a.dt, a.rdt, ns);
+                            }
                             cnt[k] += T >= ns ? 1 : 0;
                         }
                         const float f = Fr[T];
@@ -940,17 +972,20 @@ __global__ void k_selftest_sqrt(const float *__restrict__ x, int64_t n, float *_
     if (i < n) out_hw[i] = __builtin_amdgcn_sqrtf(x[i]);
 }
 
-/* For in[i] = (dIS2, dIR2, t2, v2): q_fast, the reference Q = RN(dsr/dt)
- * (float, before the floor), and the T each path selects in the kernel. */
+/* For in[i] = (dIS2, dIR2, t, v2) (t a sample index held as a float):
+ * q_fast of the migration kernel, the reference Q = RN(dsr/dt) (float,
+ * before the floor), and the T each selects; t2 is the table's value. */
 __global__ void k_selftest_traveltime(const float4 *__restrict__ in, int64_t n, float dt, int ns,
                                       float *__restrict__ q_fast, float *__restrict__ q_ref,
                                       int *__restrict__ t_kernel, int *__restrict__ t_ref)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float4 v = in[i];
+    const float4 in4 = in[i];
+    const int t = (int)in4.z;
+    const float4 v = make_float4(in4.x, in4.y, (float)(0.25 * t * t * dt * dt), in4.w);
     const float r = 1.0f / v.w, rdt = 1.0f / dt;
-    const float q = fast_q(v.x, v.y, r, v.z, rdt);
+    const float q = fast_qs(v.x, v.y, fast_rs(v.w, dt), fast_tq(t));
     /* the decision procedure of k_migrate */
     const float nsf = (float)ns, qc = fminf(q, nsf);
     int T = fast_T(qc);
@@ -1602,6 +1637,8 @@ static mig_fn pick_migrate(int ns, int accum, int *W, int *K, int *B)
 {
     if (ns <= 512) { *W = 1; *K = 8; return pick_accum<8, 1>(accum, B); }
     if (ns <= 1024) { *W = 2; *K = 8; return pick_accum<8, 2>(accum, B); }
+    /* (the Kahan K = 12 variant spills some registers, yet runs at 0.91x the
+     * plain one; four waves of K = 8 per column measured 0.63x) */
     if (ns <= 1536) { *W = 2; *K = 12; return pick_accum<12, 2>(accum, B); }
     if (ns <= 2048) { *W = 4; *K = 8; return pick_accum<8, 4>(accum, B); }
     *W = 4; *K = 12;

@@ -1,7 +1,7 @@
 """The numerical assumptions behind the fast traveltime path of k_migrate.
 
-The kernel computes q = dsr/dt with FMA, the hardware square root and a
-reciprocal of dt, and falls back to the reference arithmetic of
+The kernel computes q = dsr/dt with FMA and the hardware square root on
+radicands pre-scaled by 1/dt^2, and falls back to the reference arithmetic of
 Kirchhoff.c:1381-1384 whenever an integer lies within 6e-7 (q + 1) of q.
 That is exact only if |q_fast - q_ref| <= 6e-7 (q_ref + 1), which rests on
 v_sqrt_f32 being within one ulp.  Both facts are checked here.
@@ -65,10 +65,9 @@ def test_fast_traveltime_bound_and_index(cuda):
     dt = np.float32(0.004)
     ns = 1500
     t = rng.integers(0, ns, n)
-    t2 = (0.25 * t * t * np.float64(dt) * np.float64(dt)).astype(np.float32)
     d = rng.uniform(0.0, 8000.0, (n, 2)) ** 2
     v = rng.uniform(1400.0, 5000.0, n).astype(np.float32)
-    inp = np.stack([d[:, 0], d[:, 1], t2, v * v], axis=1).astype(np.float32)
+    inp = np.stack([d[:, 0], d[:, 1], t, v * v], axis=1).astype(np.float32)
     qf, qr, tk, tr = _run_traveltime(cuda, inp, dt, ns)
     err = np.abs(qf.astype(np.float64) - qr) / (qr.astype(np.float64) + 1.0)
     near = np.abs(qr - np.rint(qr)) <= EPS * (qr + 1.0)
