@@ -543,8 +543,10 @@ __device__ __forceinline__ float fast_q(float dIS2, float dIR2, float r, float t
  * t^2 dt^2 / 4), the reference's s1 = RN(t2 + RN(dIS2/v2)) within 2u of
  * t2e + dIS2/v2 (its t2 is t2e rounded once): 4u on the radicand, 2u after
  * the square root, plus 2u for v_sqrt_f32; the sum's rounding u; the
- * reference's RN(dsr) and RN(dsr/dt) 2u: |q - Q| <= 7u Q to first order,
- * inside KTM_FAST_EPS = 10u.  No multiply by 1/dt is left in the hot loop. */
+ * reference's RN(dsr) and RN(dsr/dt) 2u: |q - Q| <= 7u Q to first order;
+ * its tolerance KTM_FAST_EPS_S = 8u leaves 1u for the second-order terms.
+ * No multiply by 1/dt is left in the hot loop. */
+#define KTM_FAST_EPS_S 4.8e-7f
 __device__ __forceinline__ float fast_qs(float dIS2, float dIR2, float rs, float tq)
 {
     return __builtin_amdgcn_sqrtf(__builtin_fmaf(dIS2, rs, tq)) +
@@ -566,12 +568,12 @@ __device__ __forceinline__ float fast_tq(int t) { return (float)(0.25 * (double)
  * migration kernel clamps at ns instead, see there.) */
 __device__ __forceinline__ float fast_clamp(float q, float nsh) { return fminf(q, nsh); }
 
-/* An integer within KTM_FAST_EPS*(q+1) of the clamped q: floor(q) may
- * differ from the reference's floor(Q). */
-__device__ __forceinline__ bool fast_ambiguous(float qc)
+/* An integer within eps*(q+1) of the clamped q: floor(q) may differ from
+ * the reference's floor(Q).  eps is the bound of the fast form in use. */
+__device__ __forceinline__ bool fast_ambiguous(float qc, float eps = KTM_FAST_EPS)
 {
     const float dev = fabsf(qc - __builtin_rintf(qc));
-    return dev <= __builtin_fmaf(qc, KTM_FAST_EPS, KTM_FAST_EPS);
+    return dev <= __builtin_fmaf(qc, eps, eps);
 }
 
 /* floor of the clamped q, in [0, ns] */
@@ -749,10 +751,11 @@ k_migrate(const MigArgs a)
                          * with the ambiguous ones, where the samples that
                          * do not count towards fold are counted, so the
                          * common path has no compare for fold */
+                        constexpr float EPS = SCALED ? KTM_FAST_EPS_S : KTM_FAST_EPS;
                         const float qc = fminf(q, nsf);
                         int T = fast_T(qc);
-                        if (fast_ambiguous(qc)) {
-                            if (q - nsf >= __builtin_fmaf(q, KTM_FAST_EPS, KTM_FAST_EPS))
+                        if (fast_ambiguous(qc, EPS)) {
+                            if (q - nsf >= __builtin_fmaf(q, EPS, EPS))
                                 T = ns;   /* Q >= ns as well: past the trace */
                             else {
                                 const int tl = seg * K * KTM_WAVE + k * KTM_WAVE + lane;
@@ -989,8 +992,8 @@ __global__ void k_selftest_traveltime(const float4 *__restrict__ in, int64_t n, 
     /* the decision procedure of k_migrate */
     const float nsf = (float)ns, qc = fminf(q, nsf);
     int T = fast_T(qc);
-    if (fast_ambiguous(qc))
-        T = q - nsf >= __builtin_fmaf(q, KTM_FAST_EPS, KTM_FAST_EPS)
+    if (fast_ambiguous(qc, KTM_FAST_EPS_S))
+        T = q - nsf >= __builtin_fmaf(q, KTM_FAST_EPS_S, KTM_FAST_EPS_S)
                 ? ns : exact_T(v.x, v.y, v.z, v.w, r, dt, rdt, ns);
     /* the reference computed independently of exact_T: the compiler's
      * correctly rounded division and double square root */
