@@ -42,6 +42,7 @@
 
 #define VG_NT 256
 #define VG_WAVE 64
+#define VG_G 16         /* passing traces whose taps are loaded together */
 
 struct ktm_vgrad {
     ktm_config cfg;
@@ -155,21 +156,17 @@ struct VgArgs {
     double *part;                  /* [nxp * nchunks] */
 };
 
-/* Catmull-Rom cubic through F[T-1..T+2] at q = T + u, u in [0, 1) (taps
- * outside [0, ns) read the zero at F[ns]); returns the value, or with DERIV
- * its q-derivative */
+/* Catmull-Rom cubic through the taps f = F[T-1..T+2] at q = T + u, u in
+ * [0, 1) (taps outside [0, ns) read the zero at F[ns]); returns the value,
+ * or with DERIV its q-derivative */
 template <bool DERIV>
-__device__ __forceinline__ float cr_tap(const float *__restrict__ Fr, int T, float u, unsigned nsu)
+__device__ __forceinline__ float cr_eval(const float f[4], float u)
 {
-    const float f0 = Fr[min((unsigned)(T - 1), nsu)];
-    const float f1 = Fr[min((unsigned)T, nsu)];
-    const float f2 = Fr[min((unsigned)(T + 1), nsu)];
-    const float f3 = Fr[min((unsigned)(T + 2), nsu)];
-    const float c1 = 0.5f * (f2 - f0);
-    const float c2 = __builtin_fmaf(-0.5f, f3, __builtin_fmaf(2.0f, f2, __builtin_fmaf(-2.5f, f1, f0)));
-    const float c3 = __builtin_fmaf(1.5f, f1 - f2, 0.5f * (f3 - f0));
+    const float c1 = 0.5f * (f[2] - f[0]);
+    const float c2 = __builtin_fmaf(-0.5f, f[3], __builtin_fmaf(2.0f, f[2], __builtin_fmaf(-2.5f, f[1], f[0])));
+    const float c3 = __builtin_fmaf(1.5f, f[1] - f[2], 0.5f * (f[3] - f[0]));
     if (DERIV) return __builtin_fmaf(u, __builtin_fmaf(3.0f * u, c3, 2.0f * c2), c1);
-    return __builtin_fmaf(u, __builtin_fmaf(u, __builtin_fmaf(u, c3, c2), c1), f1);
+    return __builtin_fmaf(u, __builtin_fmaf(u, __builtin_fmaf(u, c3, c2), c1), f[1]);
 }
 
 /* One pass over the traces of class h for this lane's sample t.  The
@@ -202,29 +199,56 @@ __device__ __forceinline__ float vg_class_pass(const VgArgs &a, int lo, int hi, 
         }
         uint64_t m = __ballot(pass);
         while (m) {
-            const int j = __builtin_ctzll(m);
-            m &= m - 1ull;
-            const float A = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ga), j));
-            const float B = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(gb), j));
-            const int tj = __builtin_amdgcn_readlane(ti, j);
-            const float *Fr = a.prep + (int64_t)tj * a.row;
-            const float ar = A * r, br = B * r;
-            const float u1 = fmaxf(ar + t2, 1.0e-30f), u2 = fmaxf(br + t2, 1.0e-30f);
-            const float i1 = __builtin_amdgcn_rsqf(u1), i2 = __builtin_amdgcn_rsqf(u2);
-            const float w1 = __builtin_amdgcn_rcpf(fmaxf(__builtin_fmaf(u1, i1, th), 1.0e-30f));
-            const float w2 = __builtin_amdgcn_rcpf(fmaxf(__builtin_fmaf(u2, i2, th), 1.0e-30f));
-            const float d = fminf(__builtin_fmaf(ar, w1, br * w2) * a.rdt, dmax);
-            const float Df = floorf(d);
-            const int T = t + (int)Df;
-            const float u = d - Df;
-            if (FWD) {
-                const float y = cr_tap<false>(Fr, T, u, nsu) - cmp;
-                const float s = acc + y;
-                cmp = (s - acc) - y;
-                acc = s;
-            } else {
-                const float dp = cr_tap<true>(Fr, T, u, nsu);
-                acc = __builtin_fmaf(dp, __builtin_fmaf(A, i1, B * i2), acc);
+            /* up to VG_G passing traces at a time, in order: their taps are
+             * loaded together so that the loads overlap (a lone trace's four
+             * loads would leave the wave waiting on L2 latency); slots past
+             * the last passing trace repeat the first and are not added */
+            int jc[VG_G], nb = 0;
+#pragma unroll
+            for (int c = 0; c < VG_G; ++c) {
+                if (m) {
+                    jc[c] = __builtin_ctzll(m);
+                    m &= m - 1ull;
+                    nb = c + 1;
+                } else {
+                    jc[c] = jc[0];
+                }
+            }
+            float f[VG_G][4], uu[VG_G], dqv[VG_G];
+#pragma unroll
+            for (int c = 0; c < VG_G; ++c) {
+                const int j = jc[c];
+                const float A = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ga), j));
+                const float B = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(gb), j));
+                const int tj = __builtin_amdgcn_readlane(ti, j);
+                const float *Fr = a.prep + (int64_t)tj * a.row;
+                const float ar = A * r, br = B * r;
+                const float u1 = fmaxf(ar + t2, 1.0e-30f), u2 = fmaxf(br + t2, 1.0e-30f);
+                const float i1 = __builtin_amdgcn_rsqf(u1), i2 = __builtin_amdgcn_rsqf(u2);
+                const float w1 = __builtin_amdgcn_rcpf(fmaxf(__builtin_fmaf(u1, i1, th), 1.0e-30f));
+                const float w2 = __builtin_amdgcn_rcpf(fmaxf(__builtin_fmaf(u2, i2, th), 1.0e-30f));
+                const float d = fminf(__builtin_fmaf(ar, w1, br * w2) * a.rdt, dmax);
+                const float Df = floorf(d);
+                const int T = t + (int)Df;
+                uu[c] = d - Df;
+                dqv[c] = __builtin_fmaf(A, i1, B * i2);
+                f[c][0] = Fr[min((unsigned)(T - 1), nsu)];
+                f[c][1] = Fr[min((unsigned)T, nsu)];
+                f[c][2] = Fr[min((unsigned)(T + 1), nsu)];
+                f[c][3] = Fr[min((unsigned)(T + 2), nsu)];
+            }
+#pragma unroll
+            for (int c = 0; c < VG_G; ++c) {
+                if (c < nb) {
+                    if (FWD) {
+                        const float y = cr_eval<false>(f[c], uu[c]) - cmp;
+                        const float s = acc + y;
+                        cmp = (s - acc) - y;
+                        acc = s;
+                    } else {
+                        acc = __builtin_fmaf(cr_eval<true>(f[c], uu[c]), dqv[c], acc);
+                    }
+                }
             }
         }
     }
