@@ -273,3 +273,92 @@ def vgrad(cfg, ns, dt, F, coords, V, cig=False, grad=True):
                         V.shape[1], _vpn(V), None if c is None else _vpn(c),
                         None if d is None else _vpn(d))
     return J, d, c
+
+
+# ---------------------------------------------------------------- EOM ----
+class OrcEomParams(C.Structure):
+    _fields_ = [("ox", C.c_double), ("oy", C.c_double), ("angle_e_to_n", C.c_double),
+                ("oxcsp", C.c_double), ("oycsp", C.c_double), ("excsp", C.c_double),
+                ("eycsp", C.c_double), ("scale_xy", C.c_double), ("dcsp_x", C.c_double),
+                ("dcsp_y", C.c_double), ("dh", C.c_double), ("aperture", C.c_double),
+                ("low_pass_cutoff", C.c_double), ("nh", C.c_int32), ("oh", C.c_int32),
+                ("jobs_x", C.c_int32), ("jobs_y", C.c_int32)]
+
+
+class OrcEomGrid(C.Structure):
+    _fields_ = [("Nx", C.c_int32), ("Ny", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32),
+                ("no_job", C.c_int32), ("sinA", C.c_float), ("cosA", C.c_float),
+                ("start_x", C.c_float), ("start_y", C.c_float)]
+
+
+_PROTOS.update({
+    "orc_eom_grid_init": (None, [C.POINTER(OrcEomParams), C.POINTER(OrcEomGrid)]),
+    "orc_eom_filters": (None, [_i32, _f32, C.c_double, _vp, _vp]),
+    "orc_eom_job": (_i64, [C.POINTER(OrcEomParams), _i32, _f32, _i32, _vp, _vp, _i64, _vp, _vp,
+                           _vp]),
+    "orc_eom_post": (None, [_i32, _f32, _vp, _vp, _vp, _vp, _i64, _vp]),
+})
+_libs.clear()
+
+
+def eom_params(cfg) -> OrcEomParams:
+    p = OrcEomParams()
+    for name, _ in OrcEomParams._fields_:
+        setattr(p, name, getattr(cfg, name))
+    return p
+
+
+def eom_grid(cfg) -> OrcEomGrid:
+    g = OrcEomGrid()
+    p = eom_params(cfg)
+    lib().orc_eom_grid_init(C.byref(p), C.byref(g))
+    return g
+
+
+def eom_filters(cfg, ns: int, dt: float):
+    lp = np.zeros(ns, np.float32)
+    rr = np.zeros(ns, np.float32)
+    lib().orc_eom_filters(ns, dt, float(cfg.low_pass_cutoff), _vpn(lp), _vpn(rr))
+    return lp, rr
+
+
+def eom_job(cfg, ns, dt, job, traces, coords, vel):
+    """(csp, fold, evaluated) of one job slab [nx][ny][nh][ns]."""
+    p, g = eom_params(cfg), eom_grid(cfg)
+    traces = np.ascontiguousarray(traces, np.float32)
+    coords = np.ascontiguousarray(coords, np.int32)
+    vel = np.ascontiguousarray(vel, np.float32)
+    shape = (g.nx, g.ny, cfg.nh, ns)
+    csp = np.zeros(shape, np.float32)
+    fold = np.zeros(shape, np.float32)
+    n = lib().orc_eom_job(C.byref(p), ns, dt, job, _vpn(traces), _vpn(coords), traces.shape[0],
+                          _vpn(vel), _vpn(csp), _vpn(fold))
+    if n < 0:
+        raise ValueError("oracle: bad job")
+    return csp, fold, int(n)
+
+
+def eom_all(cfg, ns, dt, traces, coords, vel):
+    """All jobs into the padded gather volume [nx*jobs_x][ny*jobs_y][nh][ns]."""
+    g = eom_grid(cfg)
+    out_c = np.zeros((g.nx * cfg.jobs_x, g.ny * cfg.jobs_y, cfg.nh, ns), np.float32)
+    out_f = np.zeros_like(out_c)
+    total = 0
+    for job in range(g.no_job):
+        c, f, n = eom_job(cfg, ns, dt, job, traces, coords, vel)
+        xs, ys = job // cfg.jobs_y, job % cfg.jobs_y
+        out_c[xs * g.nx:(xs + 1) * g.nx, ys * g.ny:(ys + 1) * g.ny] = c
+        out_f[xs * g.nx:(xs + 1) * g.nx, ys * g.ny:(ys + 1) * g.ny] = f
+        total += n
+    return out_c, out_f, total
+
+
+def eom_post(csp, fold, dt, lowpass, rootrho):
+    ns = csp.shape[-1]
+    c = np.ascontiguousarray(csp.reshape(-1, ns), np.float32)
+    f = np.ascontiguousarray(fold.reshape(-1, ns), np.float32)
+    out = np.zeros_like(c)
+    lib().orc_eom_post(ns, dt, _vpn(np.ascontiguousarray(lowpass, np.float32)),
+                       _vpn(np.ascontiguousarray(rootrho, np.float32)), _vpn(c), _vpn(f),
+                       c.shape[0], _vpn(out))
+    return out.reshape(csp.shape)

@@ -761,3 +761,160 @@ double orc_vgrad(int ns, double dt, double aperture, int nh, int nxp, const floa
     free(v); free(I); free(S); free(gv);
     return J;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Equivalent-offset migration: common-scatter-point gathers (EOM.c csp())   */
+/* ------------------------------------------------------------------------ */
+#define ORC_EOM_PI 3.14159265359 /* EOM.c:124 */
+
+void orc_eom_grid_init(const orc_eom_params *p, orc_eom_grid *g)
+{
+    /* EOM.c:279-320: Nx = ceil((excsp-oxcsp)/dcspX) (1 when dcspX == 0) */
+    memset(g, 0, sizeof *g);
+    g->Nx = p->dcsp_x == 0.0 ? 1 : (int)ceil((p->excsp - p->oxcsp) / p->dcsp_x);
+    g->Ny = p->dcsp_y == 0.0 ? 1 : (int)ceil((p->eycsp - p->oycsp) / p->dcsp_y);
+    if (g->Nx == 0) g->Nx = 1;
+    if (g->Ny == 0) g->Ny = 1;
+    g->sinA = (float)sin(p->angle_e_to_n * ORC_EOM_PI / 180.0);
+    g->cosA = (float)cos(p->angle_e_to_n * ORC_EOM_PI / 180.0);
+    g->start_x = (float)(g->cosA * (p->oxcsp - p->ox) + g->sinA * (p->oycsp - p->oy));
+    g->start_y = (float)(-g->sinA * (p->oxcsp - p->ox) + g->cosA * (p->oycsp - p->oy));
+    g->no_job = p->jobs_x * p->jobs_y;
+    g->nx = (int)ceil(1.0 * g->Nx / p->jobs_x);
+    g->ny = (int)ceil(1.0 * g->Ny / p->jobs_y);
+    if (g->nx == 0) g->nx = 1;
+    if (g->ny == 0) g->ny = 1;
+}
+
+void orc_eom_filters(int ns, float dt, double cutoff, float *lowpass, float *rootrho)
+{
+    /* EOM.c:759-845; calloc'd arrays mirrored from ns/2 upwards, so for odd
+     * ns the bins ns/2 and ns/2+1 read a not-yet-set zero */
+    const float fs = (float)(2.0 / dt / ns);
+    const float fmax = (float)cutoff;
+    float *a = (float *)calloc((size_t)ns, sizeof(float));
+    float *r = (float *)calloc((size_t)ns, sizeof(float));
+    for (int i = 0; i <= ns / 2; ++i) {
+        const float f = i * fs;
+        a[i] = (float)(1.0 / sqrt(1.0 + f * f * f * f / fmax / fmax / fmax / fmax));
+        r[i] = (float)sqrt(i * fs / 2.0);
+    }
+    for (int i = ns / 2; i < ns; ++i) {
+        a[i] = a[ns - i];
+        r[i] = r[ns - i];
+    }
+    if (lowpass) memcpy(lowpass, a, sizeof(float) * (size_t)ns);
+    if (rootrho) memcpy(rootrho, r, sizeof(float) * (size_t)ns);
+    free(a);
+    free(r);
+}
+
+int64_t orc_eom_job(const orc_eom_params *p, int ns, float dt, int job, const float *traces,
+                    const int32_t *coords, int64_t ntr, const float *vel, float *csp, float *fold)
+{
+    orc_eom_grid g;
+    orc_eom_grid_init(p, &g);
+    if (job < 0 || job >= g.no_job) return -1;
+    const int nx = g.nx, ny = g.ny, nh = p->nh;
+    const int xs = job / p->jobs_y, ys = job % p->jobs_y;
+    float *v2 = (float *)malloc(sizeof(float) * (size_t)ns);
+    float *t2d = (float *)malloc(sizeof(float) * (size_t)ns);
+    for (int t = 0; t < ns; ++t) {
+        v2[t] = vel[t] * vel[t];                                  /* EOM.c:877 */
+        t2d[t] = (float)(sqrt((double)(t * dt)) * t * dt);        /* EOM.c:1196 */
+    }
+    const float aperture2 = (float)(p->aperture * p->aperture);   /* EOM.c:1176 */
+    const size_t n = (size_t)nx * ny * nh * ns;
+    for (size_t k = 0; k < n; ++k) { csp[k] = 0.0f; fold[k] = 1.0f; }  /* EOM.c:1202-1211 */
+    int64_t evaluated = 0;
+    for (int64_t i = 0; i < ntr; ++i) {
+        const uint32_t *w = (const uint32_t *)(coords + 4 * i);
+        /* EOM.c:1245-1256: scaleXY * htonl(word) - ox, rotated */
+        float io = (float)(p->scale_xy * w[0] - p->ox), jo = (float)(p->scale_xy * w[1] - p->oy);
+        const float sxt = g.cosA * io + g.sinA * jo;
+        const float syt = -g.sinA * io + g.cosA * jo;
+        io = (float)(p->scale_xy * w[2] - p->ox);
+        jo = (float)(p->scale_xy * w[3] - p->oy);
+        const float gxt = g.cosA * io + g.sinA * jo;
+        const float gyt = -g.sinA * io + g.cosA * jo;
+        const float h2 = (float)(0.25 * ((sxt - gxt) * (sxt - gxt) + (syt - gyt) * (syt - gyt)));
+        const float cmpX = (float)(0.5 * (sxt + gxt));
+        const float cmpY = (float)(0.5 * (syt + gyt));
+        const float *tr = traces + i * (int64_t)ns;
+        for (int xc = 0; xc < nx; ++xc)
+            for (int yc = 0; yc < ny; ++yc) {
+                /* EOM.c:1261-1270 */
+                const float cspX = (float)(g.start_x + (xc + xs * nx) * p->dcsp_x);
+                const float cspY = (float)(g.start_y + (yc + ys * ny) * p->dcsp_y);
+                const float x2 = (cmpX - cspX) * (cmpX - cspX) + (cmpY - cspY) * (cmpY - cspY);
+                const float dist = x2 + h2;
+                if (!(dist < aperture2)) continue;
+                const int talpha = (int)round(2.0 * sqrt((double)(h2 / v2[0])) / dt);  /* :1272 */
+                if (talpha < ns) evaluated += ns - talpha;
+                float *cs = csp + ((size_t)xc * ny + yc) * nh * ns;
+                float *fo = fold + ((size_t)xc * ny + yc) * nh * ns;
+                for (int t = talpha; t < ns; ++t) {
+                    /* EOM.c:1285-1306 */
+                    const float crossterm2 =
+                        (float)((4.0 * x2 * h2) / (double)((float)t * dt * (float)t * dt * v2[t]));
+                    const float eq = (float)sqrt((double)(x2 + h2 - crossterm2));
+                    const double qo = (double)(eq - (float)p->oh) / p->dh;
+                    const double fl = floor(qo);
+                    /* off < nh-1 && off >= 0; a NaN offset (undefined in the
+                     * reference's int conversion) contributes nothing */
+                    if (!(fl >= 0.0 && fl < (double)(nh - 1))) continue;
+                    const int off = (int)fl;
+                    const float tc = (float)(2.25 * eq / sqrt((double)v2[t]));
+                    const float scale3 =
+                        (float)(0.5 + atan(200.0 * (double)((float)t * dt - tc) / 0.080) / ORC_EOM_PI);
+                    const float scale2 = (float)(qo - off);
+                    const float scale1 = (float)(1.0 - scale2);
+                    const float scaleT = scale3;
+                    float *c0 = cs + (size_t)off * ns, *f0 = fo + (size_t)off * ns;
+                    c0[t] = c0[t] + scale1 * scaleT * tr[t] / t2d[t];
+                    f0[t] = (float)(f0[t] + 1.0);
+                    c0[ns + t] = c0[ns + t] + scale2 * scaleT * tr[t] / t2d[t];
+                    f0[ns + t] = (float)(f0[ns + t] + 1.0);
+                }
+            }
+    }
+    free(v2);
+    free(t2d);
+    return evaluated;
+}
+
+void orc_eom_post(int ns, float dt, const float *lowpass, const float *rootrho, const float *csp,
+                  const float *fold, int64_t ntraces, float *out)
+{
+    /* EOM.c:1386-1413: x = (csp/fold) dt t, FFT, low-pass, (1+i) sqrt-rho,
+     * inverse FFT, real part / ns; the spectrum passes through float a,b,c,d */
+    dft_tab d;
+    dft_init(&d, ns);
+    double *x = (double *)malloc(sizeof(double) * (size_t)ns);
+    double *re = (double *)malloc(sizeof(double) * (size_t)ns);
+    double *im = (double *)malloc(sizeof(double) * (size_t)ns);
+    double *y = (double *)malloc(sizeof(double) * (size_t)ns);
+    for (int64_t j = 0; j < ntraces; ++j) {
+        const float *c = csp + j * (int64_t)ns, *f = fold + j * (int64_t)ns;
+        x[0] = 0.0;
+        for (int t = 1; t < ns; ++t) x[t] = (c[t] / f[t]) * dt * t;
+        dft_fwd_real(&d, x, re, im);
+        for (int k = 0; k < ns; ++k) {
+            float a = lowpass[k], b = 0.0f, cc = (float)re[k], dd = (float)im[k];
+            re[k] = a * cc - b * dd;
+            im[k] = a * dd + b * cc;
+            a = rootrho[k];
+            b = rootrho[k];
+            cc = (float)re[k];
+            dd = (float)im[k];
+            re[k] = a * cc - b * dd;
+            im[k] = a * dd + b * cc;
+        }
+        dft_bwd_re(&d, re, im, y);
+        float *o = out + j * (int64_t)ns;
+        o[0] = 0.0f;
+        for (int t = 1; t < ns; ++t) o[t] = (float)(y[t] / ns);
+    }
+    free(x); free(re); free(im); free(y);
+    dft_free(&d);
+}

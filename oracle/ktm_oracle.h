@@ -26,6 +26,8 @@
  *   orc_poststack         direct DFT in exact rational/np.fft agreement,
  *                         linear-phase impulse response
  *   orc_segy_*            byte layout of READ_SEGY.c:29-115 (field order)
+ *   orc_eom_*             brute-force pure-Python loop, equivalent-offset closed
+ *                         forms, linearity, fold count, np.fft for the filter
  *   orc_vgrad             FP64 central differences of J at every node, spline
  *                         partition of unity, Catmull-Rom reproduction of
  *                         quadratics, zero-offset identity I = F
@@ -167,6 +169,39 @@ double orc_vgrad(int ns, double dt, double aperture, int nh, int nxp, const floa
 /* v(t, X) of the spline above, [nxp][ns]. */
 void  orc_vgrad_velocity(int ns, int nxp, int nt_nodes, int nx_nodes, const double *V,
                          double *v);
+
+/* ---- Equivalent-offset migration (EOM.c, csp(), the common-scatter-point
+ * gather operator).  Macros of EOM.c:60-99 with their types. ---- */
+typedef struct orc_eom_params {
+    double  ox, oy, angle_e_to_n;        /* EOM.c:66-68 */
+    double  oxcsp, oycsp, excsp, eycsp;  /* EOM.c:73-76 */
+    double  scale_xy;                    /* EOM.c:77 */
+    double  dcsp_x, dcsp_y, dh;          /* EOM.c:79-81 */
+    double  aperture, low_pass_cutoff;   /* EOM.c:85-86 */
+    int32_t nh, oh;                      /* EOM.c:82, 99 (oh is an integer macro) */
+    int32_t jobs_x, jobs_y;              /* EOM.c:91-92 */
+} orc_eom_params;
+
+typedef struct orc_eom_grid {
+    int32_t Nx, Ny, nx, ny, no_job;
+    float   sinA, cosA, start_x, start_y;  /* EOM.c:282-286 */
+} orc_eom_grid;
+
+void  orc_eom_grid_init(const orc_eom_params *p, orc_eom_grid *g);
+/* EOM.c:759-845: low-pass and sqrt-rho spectra (rootrhoR == rootrhoI). */
+void  orc_eom_filters(int ns, float dt, double cutoff, float *lowpass, float *rootrho);
+/* EOM.c:1157-1316 for one job: csp/fold [nx][ny][nh][ns] (csp = 0, fold = 1
+ * first), traces[ntr][ns] IEEE samples, coords[ntr][4] header words 18-21,
+ * vel[ns] the RMS velocity (squared inside, EOM.c:877).  Returns the number
+ * of evaluated samples (sum over passing (trace, CSP) of ns - talpha), -1 on
+ * a bad job. */
+int64_t orc_eom_job(const orc_eom_params *p, int ns, float dt, int job, const float *traces,
+                    const int32_t *coords, int64_t ntr, const float *vel, float *csp,
+                    float *fold);
+/* EOM.c:1386-1413 for ntraces gathers traces: out = Re IFFT(rootrho (1+i)
+ * lowpass FFT((csp/fold) dt t)) / ns, out[0] = 0. */
+void  orc_eom_post(int ns, float dt, const float *lowpass, const float *rootrho,
+                   const float *csp, const float *fold, int64_t ntraces, float *out);
 
 #ifdef __cplusplus
 }
