@@ -263,6 +263,53 @@ int64_t ktm_vgrad_pairs(const ktm_vgrad *vg);
 int ktm_image_records(const ktm_plan *plan, const float *image, int job, void *records,
                       void *stream);
 
+/* ---- equivalent-offset migration (EOM.c): common-scatter-point gathers ----
+ * The macros of EOM.c:60-99 with their types (oh is an integer macro). */
+typedef struct ktm_eom_config {
+    double  ox, oy, angle_e_to_n;        /* EOM.c:66-68 */
+    double  oxcsp, oycsp, excsp, eycsp;  /* EOM.c:73-76 */
+    double  scale_xy;                    /* EOM.c:77 */
+    double  dcsp_x, dcsp_y, dh;          /* EOM.c:79-81 */
+    double  aperture, low_pass_cutoff;   /* EOM.c:85-86 */
+    int32_t nh, oh;                      /* EOM.c:82, 99 */
+    int32_t jobs_x, jobs_y;              /* EOM.c:91-92 */
+    int32_t ntblock, reserved;           /* EOM.c:89 */
+    double  velocity_start, velocity_end;/* EOM.c:61-62 */
+} ktm_eom_config;
+void ktm_eom_config_default(ktm_eom_config *cfg);
+/* main()'s EOM setup (EOM.c:279-320) and the filters of EOM.c:759-845:
+ * lowpass and rootrho (rootrhoR == rootrhoI) get ns floats each if non-NULL. */
+typedef struct ktm_eom_setup {
+    int32_t ns;
+    float   dt;
+    int32_t Nx, Ny, nx, ny, no_job, nxp, nyp, nh;
+    float   sinA, cosA, start_x, start_y;   /* EOM.c:282-286 */
+} ktm_eom_setup;
+int ktm_eom_setup_init(const ktm_eom_config *cfg, int ns, float dt, ktm_eom_setup *out,
+                       float *lowpass, float *rootrho);
+typedef struct ktm_eom_plan ktm_eom_plan;
+/* vel: ns RMS velocities (squared inside, EOM.c:877; the reference's own
+ * model is velocity_start at every sample, EOM.c:871-875). */
+int ktm_eom_plan_create(ktm_eom_plan **plan, const ktm_eom_config *cfg, int ns, float dt,
+                        const float *vel);
+int ktm_eom_plan_destroy(ktm_eom_plan *plan);
+int ktm_eom_plan_setup(const ktm_eom_plan *plan, ktm_eom_setup *out);
+/* Elements of one gather volume csp/fold: nxp*nyp*nh*ns, [X][Y][offset bin][t]. */
+int64_t ktm_eom_image_size(const ktm_eom_plan *plan);
+/* csp = 0, fold = 1 (EOM.c:1202-1211), device. */
+int ktm_eom_reset(const ktm_eom_plan *plan, float *csp, float *fold, void *stream);
+/* Bin a block of traces (device: traces[ntr][ns] IEEE samples, coords[ntr][4]
+ * header words 18-21) into the gathers of every job slab (job < 0) or one,
+ * continuing from their contents in trace order: the trace loop of
+ * EOM.c:1226-1316.  counter (device uint64, may be NULL) accumulates the
+ * evaluated samples, sum over passing (trace, CSP) of ns - talpha. */
+int ktm_eom_block(const ktm_eom_plan *plan, const float *traces, const int32_t *coords,
+                  int64_t ntr, int job, float *csp, float *fold, uint64_t *counter, void *stream);
+/* out = Re IFFT(rootrho (1+i) lowpass FFT((csp/fold) dt t)) / ns per gather
+ * trace, out[..][0] = 0 (EOM.c:1386-1413), device. */
+int ktm_eom_post(const ktm_eom_plan *plan, const float *csp, const float *fold, float *out,
+                 void *stream);
+
 /* Host convenience: whole reference job loop for all jobs (Kirchhoff.c:
  * 347-350 on one rank): prep + migrate in blocks of cfg->ntblock + post-stack.
  * Host arrays; km/fold/image are [nxp][nyp][nh][ns] (any may be NULL). */

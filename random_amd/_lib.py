@@ -76,6 +76,24 @@ class KtmStats(C.Structure):
                 ("total_ms", C.c_double)]
 
 
+class KtmEomConfig(C.Structure):
+    _fields_ = [("ox", C.c_double), ("oy", C.c_double), ("angle_e_to_n", C.c_double),
+                ("oxcsp", C.c_double), ("oycsp", C.c_double), ("excsp", C.c_double),
+                ("eycsp", C.c_double), ("scale_xy", C.c_double), ("dcsp_x", C.c_double),
+                ("dcsp_y", C.c_double), ("dh", C.c_double), ("aperture", C.c_double),
+                ("low_pass_cutoff", C.c_double), ("nh", C.c_int32), ("oh", C.c_int32),
+                ("jobs_x", C.c_int32), ("jobs_y", C.c_int32), ("ntblock", C.c_int32),
+                ("reserved", C.c_int32), ("velocity_start", C.c_double),
+                ("velocity_end", C.c_double)]
+
+
+class KtmEomSetup(C.Structure):
+    _fields_ = [("ns", C.c_int32), ("dt", C.c_float), ("Nx", C.c_int32), ("Ny", C.c_int32),
+                ("nx", C.c_int32), ("ny", C.c_int32), ("no_job", C.c_int32), ("nxp", C.c_int32),
+                ("nyp", C.c_int32), ("nh", C.c_int32), ("sinA", C.c_float), ("cosA", C.c_float),
+                ("start_x", C.c_float), ("start_y", C.c_float)]
+
+
 GEOM_FIELDS = ("sxt", "syt", "gxt", "gyt", "cmpx", "cmpy", "off", "scale")
 
 _vp, _i64, _i32, _f32 = C.c_void_p, C.c_int64, C.c_int, C.c_float
@@ -115,6 +133,16 @@ _PROTOS = {
     "ktm_poststack_adjoint": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "ktm_ingest": (_i32, [_vp, _vp, _i64, _i32, _i32, _i32, _vp, _vp, _vp, _vp]),
     "ktm_segy_decode": (_i32, [_vp, _i64, _i32, _i32, _vp, _vp, _vp]),
+    "ktm_eom_config_default": (None, [C.POINTER(KtmEomConfig)]),
+    "ktm_eom_setup_init": (_i32, [C.POINTER(KtmEomConfig), _i32, _f32, C.POINTER(KtmEomSetup),
+                                  _vp, _vp]),
+    "ktm_eom_plan_create": (_i32, [C.POINTER(_vp), C.POINTER(KtmEomConfig), _i32, _f32, _vp]),
+    "ktm_eom_plan_destroy": (_i32, [_vp]),
+    "ktm_eom_plan_setup": (_i32, [_vp, C.POINTER(KtmEomSetup)]),
+    "ktm_eom_image_size": (_i64, [_vp]),
+    "ktm_eom_reset": (_i32, [_vp, _vp, _vp, _vp]),
+    "ktm_eom_block": (_i32, [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp]),
+    "ktm_eom_post": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "ktm_segy_encode": (_i32, [_vp, _i64, _i32, _i32, _vp, _vp, _vp]),
     "ktm_image_records": (_i32, [_vp, _vp, _i32, _vp, _vp]),
     "ktm_device_alloc": (_i32, [C.POINTER(_vp), C.c_size_t]),

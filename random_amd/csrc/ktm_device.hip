@@ -34,6 +34,7 @@
 #include <string.h>
 
 #include "ktm_internal.h"
+#include "ktm_devmath.h"
 
 #define KTM_WAVE 64
 
@@ -470,35 +471,6 @@ struct MigArgs {
     int tiles_x, ntiles, tx_cols, ty_cols;
     int rx0, ry0, rx1, ry1;          /* migrated region [rx0,rx1) x [ry0,ry1) */
 };
-
-/* RN(a/b) from rb = RN(1/b): q = RN(a rb) is within one ulp of a/b, the
- * residual a - b q is exact under FMA, and one correction step rounds
- * correctly (Markstein).  Checked against true division on 5e8 pairs,
- * exhaustively over [2^-10, 32) for b = 0.004 (the DSR / dt division). */
-__device__ __forceinline__ float div_cr(float a, float b, float rb)
-{
-    const float q = a * rb;
-    const float r = __builtin_fmaf(-q, b, a);
-    return __builtin_fmaf(r, rb, q);
-}
-
-/* Correctly rounded double square root of 0 or a positive normal float:
- * reciprocal square-root seed and the Goldschmidt/Newton correction used by
- * the AMDGPU f64 sqrt expansion, without its denormal scaling (inputs are
- * >= 2^-149 or exactly 0). */
-__device__ __forceinline__ double sqrt_cr(double x)
-{
-    const double y = __builtin_amdgcn_rsq(x);
-    double g = x * y, h = 0.5 * y;
-    const double r = __builtin_fma(-h, g, 0.5);
-    g = __builtin_fma(g, r, g);
-    h = __builtin_fma(h, r, h);
-    double d = __builtin_fma(-g, g, x);
-    g = __builtin_fma(d, h, g);
-    d = __builtin_fma(-g, g, x);
-    g = __builtin_fma(d, h, g);
-    return x == 0.0 ? 0.0 : g;
-}
 
 /* Reference semantics of one traveltime index (Kirchhoff.c:1381-1384):
  * FP32 true divisions and sums, the two square roots in double and summed in

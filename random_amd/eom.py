@@ -11,8 +11,13 @@ gathers are then scaled by t dt / fold and filtered by the low-pass and
 """
 from __future__ import annotations
 
+import ctypes as C
 import dataclasses
 from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import KtmEomConfig, KtmEomSetup, check, lib
 
 
 @dataclass
@@ -43,10 +48,100 @@ class EomConfig:
     def replace(self, **kw) -> "EomConfig":
         return dataclasses.replace(self, **kw)
 
+    def to_c(self) -> KtmEomConfig:
+        c = KtmEomConfig()
+        for f in dataclasses.fields(self):
+            setattr(c, f.name, getattr(self, f.name))
+        return c
+
     def velocity(self, ns: int):
         """vel_t of EOM.c:871-875: velocityStart + (End - Start) * (i/ns) with
         the integer division of the source, i.e. velocityStart for every
         sample (i < ns)."""
-        import numpy as np
         return np.array([self.velocity_start + (self.velocity_end - self.velocity_start) * (i // ns)
                          for i in range(ns)], np.float32)
+
+
+def eom_setup(cfg: EomConfig, ns: int, dt: float):
+    """(setup dict, lowpass float32[ns], rootrho float32[ns]) without a device."""
+    s = KtmEomSetup()
+    lp = np.zeros(ns, np.float32)
+    rr = np.zeros(ns, np.float32)
+    c = cfg.to_c()
+    check(lib().ktm_eom_setup_init(C.byref(c), ns, dt, C.byref(s), lp.ctypes.data_as(C.c_void_p),
+                                   rr.ctypes.data_as(C.c_void_p)), "ktm_eom_setup_init")
+    return {f: getattr(s, f) for f, _ in KtmEomSetup._fields_}, lp, rr
+
+
+class EomPlan:
+    """Device-resident setup of one EOM configuration (``ktm_eom_plan``)."""
+
+    def __init__(self, cfg: EomConfig, ns: int, dt: float, vel=None):
+        import torch  # noqa: F401  (device context comes from torch)
+        self.cfg = cfg
+        self.ns, self.dt = int(ns), float(np.float32(dt))
+        vel = cfg.velocity(ns) if vel is None else np.ascontiguousarray(vel, np.float32)
+        if vel.shape != (self.ns,):
+            raise ValueError("vel must have ns samples")
+        self._c = cfg.to_c()
+        h = C.c_void_p()
+        check(lib().ktm_eom_plan_create(C.byref(h), C.byref(self._c), self.ns, self.dt,
+                                        vel.ctypes.data_as(C.c_void_p)), "ktm_eom_plan_create")
+        self._h = h
+        s = KtmEomSetup()
+        check(lib().ktm_eom_plan_setup(self._h, C.byref(s)), "ktm_eom_plan_setup")
+        self.setup = {f: getattr(s, f) for f, _ in KtmEomSetup._fields_}
+        self.shape = (s.nxp, s.nyp, s.nh, self.ns)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().ktm_eom_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _stream():
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+
+    def new_gathers(self, device="cuda"):
+        """csp = 0, fold = 1 on the device (EOM.c:1202-1211)."""
+        import torch
+        csp = torch.empty(self.shape, dtype=torch.float32, device=device)
+        fold = torch.empty_like(csp)
+        check(lib().ktm_eom_reset(self._h, csp.data_ptr(), fold.data_ptr(), self._stream()),
+              "ktm_eom_reset")
+        return csp, fold
+
+    def block(self, traces, coords, csp, fold, job: int = -1, counter=None) -> None:
+        """Bin one block of traces (device tensors) into the gathers."""
+        import torch
+        assert traces.dtype == torch.float32 and traces.is_contiguous() and traces.shape[1] == self.ns
+        assert coords.dtype == torch.int32 and coords.is_contiguous() and coords.shape == (traces.shape[0], 4)
+        check(lib().ktm_eom_block(self._h, traces.data_ptr(), coords.data_ptr(), traces.shape[0],
+                                  job, csp.data_ptr(), fold.data_ptr(),
+                                  None if counter is None else counter.data_ptr(), self._stream()),
+              "ktm_eom_block")
+
+    def migrate(self, traces, coords, block: int | None = None, job: int = -1):
+        """All traces in blocks of ntblock: (csp, fold, evaluated samples)."""
+        import torch
+        block = block or self.cfg.ntblock
+        csp, fold = self.new_gathers(traces.device)
+        counter = torch.zeros(1, dtype=torch.int64, device=traces.device)
+        for i0 in range(0, traces.shape[0], block):
+            self.block(traces[i0:i0 + block], coords[i0:i0 + block], csp, fold, job, counter)
+        return csp, fold, int(counter.item())
+
+    def post(self, csp, fold):
+        """The scaled, low-passed, sqrt-rho filtered gathers (EOM.c:1386-1413)."""
+        import torch
+        out = torch.empty_like(csp)
+        check(lib().ktm_eom_post(self._h, csp.data_ptr(), fold.data_ptr(), out.data_ptr(),
+                                 self._stream()), "ktm_eom_post")
+        return out
