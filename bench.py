@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--ncols", type=int, default=400)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline time")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the ingest and velocity-gradient legs")
+    ap.add_argument("--no-extra", action="store_true", help="skip the ingest, velocity-gradient and EOM legs")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3")
     ap.add_argument("--leg-steps", type=int, default=3, help="timed blocks per precision leg")
     ap.add_argument("--err-traces", type=int, default=20000,
@@ -251,6 +251,51 @@ def vgrad_leg(dev, reps=3, fd_nodes=10):
             "fd_max_rel_err": float(rel.max()), "fd_median_rel_err": float(np.median(rel))}
 
 
+def eom_leg(dev, reps=3, cpu_traces=48):
+    """Equivalent-offset migration (EOM.c csp()) of the configuration-0 line:
+    4800 traces x 1001 samples into 800 CSPs x 160 offset bins, plus the post
+    filter; the CPU oracle timed on the first traces of the same survey."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from random_amd import surveys
+    from random_amd.eom import EomPlan
+    sv, cfg = surveys.line2d_eom()
+    tr = surveys.traces_numpy(sv)
+    plan = EomPlan(cfg, sv.ns, sv.dt)
+    d_tr, d_co = torch.from_numpy(tr).to(dev), torch.from_numpy(sv.coords).to(dev)
+    plan.migrate(d_tr, d_co)
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    ms_b, ms_p = [], []
+    for _ in range(reps):
+        csp, fold = plan.new_gathers(dev)
+        counter = torch.zeros(1, dtype=torch.int64, device=dev)
+        torch.cuda.synchronize()
+        e0.record()
+        plan.block(d_tr, d_co, csp, fold, -1, counter)
+        e1.record()
+        plan.post(csp, fold)
+        e2.record()
+        torch.cuda.synchronize()
+        ms_b.append(e0.elapsed_time(e1))
+        ms_p.append(e1.elapsed_time(e2))
+    n = int(counter.item())
+    ms_b, ms_p = float(np.median(ms_b)), float(np.median(ms_p))
+    sub = slice(0, cpu_traces)
+    t0 = time.perf_counter()
+    _, _, ncpu = oracle.eom_all(cfg, sv.ns, np.float32(sv.dt), tr[sub], sv.coords[sub],
+                                cfg.velocity(sv.ns))
+    el = time.perf_counter() - t0
+    log(f"eom: binning {ms_b:.2f} ms, post {ms_p:.2f} ms, {n:.3e} samples; cpu {ncpu:.3e} in {el:.2f}s")
+    return {"workload": "configuration-0 line, 4800 traces x 1001 samples -> 800 CSPs x 160 "
+                        "offset bins of 12.5 m, aperture 2000 m, v = 2000 m/s",
+            "evaluated_samples": n, "ms_binning": round(ms_b, 2), "ms_post": round(ms_p, 2),
+            "G_samples_per_s": round(n / (ms_b * 1e-3) / 1e9, 2),
+            "cpu_oracle_G_samples_per_s": round(ncpu / el / 1e9, 4),
+            "cpu_sample": f"first {cpu_traces} traces, one thread, {el:.2f} s"}
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -365,6 +410,7 @@ def main():
     if not args.profile and not args.no_extra:
         result["ingest"] = ingest_leg(sv, plan, traces, args.block, dev)
         result["velocity_gradient"] = vgrad_leg(dev)
+        result["eom"] = eom_leg(dev)
     if not args.no_cpu and not args.profile:
         nthreads = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
         nthreads = max(1, min(nthreads, 64))

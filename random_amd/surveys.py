@@ -174,6 +174,21 @@ def line2d_vgrad(seed=20240041, **kw) -> Survey:
     return line2d(**args)
 
 
+def line2d_eom(seed=20240001):
+    """The configuration-0 line (40 shots x 120 receivers, 1001 samples) with
+    an equivalent-offset migration set-up: 800 CSPs at 6.25 m from 0 (the KTM
+    image columns), 160 offset bins of 12.5 m, aperture 2000 m, and the
+    constant velocity of EOM.c's own model (2000 m/s).  Returns (survey,
+    EomConfig)."""
+    from .eom import EomConfig
+    sv = line2d(seed=seed)
+    cfg = EomConfig(ox=0.0, oy=0.0, angle_e_to_n=0.0, oxcsp=0.0, oycsp=0.0, excsp=5000.0,
+                    eycsp=0.0, scale_xy=0.01, dcsp_x=6.25, dcsp_y=0.0, dh=12.5, nh=160,
+                    aperture=2000.0, jobs_x=1, jobs_y=1, ntblock=50000,
+                    velocity_start=2000.0, velocity_end=2000.0)
+    return sv, cfg
+
+
 def point_reflectivity(sv: Survey, image_shape) -> np.ndarray:
     """Image-domain model with one unit-amplitude spike (times the scatterer
     amplitude) at each scatterer's (column, t0 sample)."""
