@@ -309,6 +309,27 @@ int ktm_eom_block(const ktm_eom_plan *plan, const float *traces, const int32_t *
  * trace, out[..][0] = 0 (EOM.c:1386-1413), device. */
 int ktm_eom_post(const ktm_eom_plan *plan, const float *csp, const float *fold, float *out,
                  void *stream);
+/* The same on one job's gathers only, [nx][ny][nh][ns] (csp_data and fold of
+ * csp(job_id), EOM.c:1190-1211): element count, reset, block (job >= 0) and
+ * post filter, device. */
+int64_t ktm_eom_job_size(const ktm_eom_plan *plan);
+int ktm_eom_job_reset(const ktm_eom_plan *plan, float *csp, float *fold, void *stream);
+int ktm_eom_job_block(const ktm_eom_plan *plan, const float *traces, const int32_t *coords,
+                      int64_t ntr, int job, float *csp, float *fold, uint64_t *counter,
+                      void *stream);
+int ktm_eom_job_post(const ktm_eom_plan *plan, const float *csp, const float *fold, float *out,
+                     void *stream);
+/* The records of image<job>.su (EOM.c:1346-1537) from the job's post-filtered
+ * gathers post [nx][ny][nh][ns] (device): per (xc, yc, ih) in order a 240-byte
+ * header (tracl = running number, cdp = iline = CSP number, offset =
+ * ceil(oh + ih dh), sx..gy = 13..16, cdpx/cdpy = ox/oy + the rotated CSP
+ * position truncated, cline = 1, 9999 at byte 236 and, from the second record
+ * on, at 204, 220, 224, 232) and ns big-endian IEEE samples (the first 0).
+ * Only records with running number < Nx*Ny*nh + 1 are written (EOM.c:1417),
+ * a prefix of the job's: *nrec (host) receives their count.  records:
+ * device, room for nx*ny*nh*(240 + 4 ns) bytes, 4-byte aligned. */
+int ktm_eom_job_records(const ktm_eom_plan *plan, const float *post, int job, void *records,
+                        int64_t *nrec, void *stream);
 
 /* Host convenience: whole reference job loop for all jobs (Kirchhoff.c:
  * 347-350 on one rank): prep + migrate in blocks of cfg->ntblock + post-stack.
@@ -345,7 +366,10 @@ int  ktm_segy_write(const char *path, const uint8_t *file_header3600,
 int  ktm_segy_read_velocity(const char *path, const ktm_config *cfg, float *vel,
                             int *nsm, float *dtm, int64_t *nzero);
 /* Headers of the image output (Kirchhoff.c:729-814, 1437-1590). fix_count
- * writes at byte 3212 the trace count the reference meant to write. */
+ * writes at byte 3212 the trace count the reference meant to write.  A trace
+ * header other than the job file's first (xc, yc, ih) = (0, 0, 0) carries
+ * 9999 at bytes 204, 220, 224 and 232 as well as 236: the reference's `mess`
+ * keeps the value set at Kirchhoff.c:1589 for the following records. */
 void ktm_segy_file_header(const ktm_config *cfg, int ns, float dt, int fix_count,
                           uint8_t out[3600]);
 void ktm_segy_image_trace_header(const ktm_config *cfg, int ns, float dt, int job,
@@ -365,6 +389,19 @@ void kirchhoff(int job_id);
 /* Status of the last kirchhoff() call (the reference exits instead). */
 int  ktm_last_status(void);
 void ktm_globals_free(void);
+
+/* csp(job_id), the entry point of EOM.c (EOM.c:1058), with main()'s setup
+ * (EOM.c:274-345) done by ktm_eom_globals_init: count_size_file, the CSP
+ * grid, the velocity of velocity_model 0 (EOM.c:869-878), the filters, mkdir
+ * image_dir, ebdic_header.bin + binary_header.bin.  csp(job) reads the input
+ * SEG-Y (IBM) in blocks of cfg->ntblock, decodes, bins, post-filters and
+ * encodes on the device and writes image_dir/image<job>.su. */
+int  ktm_eom_globals_init(const ktm_eom_config *cfg, const char *input_data,
+                          const char *image_dir);
+void csp(int job_id);
+/* Status of the last csp() call (the reference exits instead). */
+int  ktm_eom_last_status(void);
+void ktm_eom_globals_free(void);
 
 /* ---- seeded synthetic surveys (device), for tests and benchmarks ---- */
 

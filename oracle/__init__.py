@@ -60,7 +60,9 @@ _PROTOS = {
     "orc_poststack_fp32": (None, [_i32, _f32, _vp, _vp, _vp, _i64, _vp]),
     "orc_poststack_fp64": (None, [_i32, _f32, _vp, _vp, _vp, _i64, _vp]),
     "orc_segy_trace_header": (None, [C.POINTER(OrcParams), C.POINTER(OrcGrid), _i32, _f32, _i32,
-                                     _i32, _i32, _i32, _i32, _vp]),
+                                     _i32, _i32, _i32, _i32, _i32, _vp]),
+    "orc_segy_job_headers": (None, [C.POINTER(OrcParams), C.POINTER(OrcGrid), _i32, _f32, _i32,
+                                    _vp]),
     "orc_segy_file_header": (None, [C.POINTER(OrcParams), C.POINTER(OrcGrid), _i32, _f32, _i32, _vp]),
     "orc_count_size": (_i32, [_vp, _i64, C.POINTER(_i32), C.POINTER(_f32), C.POINTER(_i32)]),
     "orc_vgrad": (C.c_double, [_i32, C.c_double, C.c_double, _i32, _i32, _vp, _f32, _vp, _vp, _vp, _i64,
@@ -208,11 +210,22 @@ def poststack(km, fold, dt, anti_alias):
     return out.reshape(km.shape)
 
 
-def segy_trace_header(cfg, ns, dt, job, xc, yc, ih, tracl) -> bytes:
+def segy_trace_header(cfg, ns, dt, job, xc, yc, ih, tracl, mess) -> bytes:
+    """One image trace header; mess is 0 for the first record of a job file
+    and 9999 for every later one (the reference's carried local)."""
     p, g = params(cfg), grid(cfg)
     out = np.zeros(240, np.uint8)
-    lib().orc_segy_trace_header(C.byref(p), C.byref(g), ns, dt, job, xc, yc, ih, tracl, _vpn(out))
+    lib().orc_segy_trace_header(C.byref(p), C.byref(g), ns, dt, job, xc, yc, ih, tracl, mess,
+                                _vpn(out))
     return out.tobytes()
+
+
+def segy_job_headers(cfg, ns, dt, job) -> np.ndarray:
+    """[nx*ny*nh][240] uint8: the headers of image<job>.su in write order."""
+    p, g = params(cfg), grid(cfg)
+    out = np.zeros((g.nx * g.ny * cfg.nh, 240), np.uint8)
+    lib().orc_segy_job_headers(C.byref(p), C.byref(g), ns, dt, job, _vpn(out))
+    return out
 
 
 def segy_file_header(cfg, ns, dt, fix_count=0) -> bytes:
@@ -297,6 +310,10 @@ _PROTOS.update({
     "orc_eom_job": (_i64, [C.POINTER(OrcEomParams), _i32, _f32, _i32, _vp, _vp, _i64, _vp, _vp,
                            _vp]),
     "orc_eom_post": (None, [_i32, _f32, _vp, _vp, _vp, _vp, _i64, _vp]),
+    "orc_eom_trace_header": (None, [C.POINTER(OrcEomParams), _i32, _f32, _i32, _i32, _i32, _i32,
+                                    _i32, _i32, _vp]),
+    "orc_eom_job_records": (_i64, [C.POINTER(OrcEomParams), _i32, _f32, _i32, _vp, _vp]),
+    "orc_eom_file_header": (None, [_i32, _f32, _vp]),
 })
 _libs.clear()
 
@@ -362,3 +379,28 @@ def eom_post(csp, fold, dt, lowpass, rootrho):
                        _vpn(np.ascontiguousarray(rootrho, np.float32)), _vpn(c), _vpn(f),
                        c.shape[0], _vpn(out))
     return out.reshape(csp.shape)
+
+
+def eom_trace_header(cfg, ns, dt, job, xc, yc, ih, it, mess) -> bytes:
+    p = eom_params(cfg)
+    out = np.zeros(240, np.uint8)
+    lib().orc_eom_trace_header(C.byref(p), ns, dt, job, xc, yc, ih, it, mess, _vpn(out))
+    return out.tobytes()
+
+
+def eom_job_records(cfg, ns, dt, job, post) -> bytes:
+    """The bytes of image<job>.su from the job's post-filtered gathers
+    [nx][ny][nh][ns] (only the records with it < Nx*Ny*nh + 1)."""
+    p = eom_params(cfg)
+    post = np.ascontiguousarray(post, np.float32)
+    out = np.zeros(post.size // ns * (240 + 4 * ns), np.uint8)
+    n = lib().orc_eom_job_records(C.byref(p), ns, dt, job, _vpn(post), _vpn(out))
+    if n < 0:
+        raise ValueError("oracle: bad job")
+    return out[:n * (240 + 4 * ns)].tobytes()
+
+
+def eom_file_header(ns, dt) -> bytes:
+    out = np.zeros(3600, np.uint8)
+    lib().orc_eom_file_header(ns, dt, _vpn(out))
+    return out.tobytes()

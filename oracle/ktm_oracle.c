@@ -569,12 +569,51 @@ static void put_be16(uint8_t *o, uint16_t v)
     o[0] = (uint8_t)(v >> 8); o[1] = (uint8_t)v;
 }
 
+/* The 240 bytes of one output trace header as the reference emits them: the
+ * fwrite sequence of Kirchhoff.c:1499-1590 (identical in EOM.c:1437-1497),
+ * 4- and 2-byte big-endian items in order.  Only the items the reference
+ * sets to something other than zero are named; every other item is the
+ * zeroed struct field (Kirchhoff.c:1140-1225).  Item indices: */
+enum {
+    H_TRACL = 0, H_CDP = 5, H_OFFSET = 11, H_SX = 21, H_SY, H_GX, H_GY,
+    H_NS = 38, H_DT = 39, H_CDPX = 71, H_CDPY, H_ILINE, H_CLINE,
+    H_MESS0 = 78, H_MESS1 = 85, H_MESS2 = 86, H_MESS3 = 89, H_END = 90, H_COUNT = 91
+};
+/* width of each item in write order: tracl..cdpt (7 x 4), trid..duse (4 x 2),
+ * offset..gwdep (8 x 4), scalel, scalco (2 x 2), sx..gy (4 x 4), counit..mute
+ * (13 x 2), ns, dt (2 x 2), gain..otrav (31 x 2), cdpx..shotpt (5 x 4), then
+ * messs, messs, mess, 6 x messs, mess, mess, messs, messs, mess, 9999 */
+static int header_item_width(int k)
+{
+    if (k < 7) return 4;
+    if (k < 11) return 2;
+    if (k < 19) return 4;
+    if (k < 21) return 2;
+    if (k < 25) return 4;
+    if (k < 71) return 2;
+    if (k < 76) return 4;
+    static const int tail[H_COUNT - 76] = {2, 2, 4, 2, 2, 2, 2, 2, 2, 4, 4, 2, 2, 4, 4};
+    return tail[k - 76];
+}
+
+static void emit_header(const int64_t item[H_COUNT], uint8_t out[240])
+{
+    int pos = 0;
+    for (int k = 0; k < H_COUNT; ++k) {
+        const int w = header_item_width(k);
+        if (w == 4) put_be32(out + pos, (int32_t)(uint32_t)item[k]);
+        else put_be16(out + pos, (uint16_t)item[k]);
+        pos += w;
+    }
+}
+
 void orc_segy_trace_header(const orc_params *p, const orc_grid *g, int ns, float dt,
-                           int job, int xc, int yc, int ih, int tracl, uint8_t out[240])
+                           int job, int xc, int yc, int ih, int tracl, int32_t mess,
+                           uint8_t out[240])
 {
     const int xslice = job / p->jobs_y, yslice = job % p->jobs_y;
     const int X = xc + xslice * g->nx, Y = yc + yslice * g->ny;
-    memset(out, 0, 240);
+    int64_t item[H_COUNT] = {0};
     /* Kirchhoff.c:1444 */
     int cdp = X + g->Nx * Y + 1;
     /* Kirchhoff.c:1482-1486 */
@@ -586,22 +625,36 @@ void orc_segy_trace_header(const orc_params *p, const orc_grid *g, int ns, float
     /* Kirchhoff.c:1492-1493: int + float - float, truncated to int */
     int cdpx = (int)((int)p->ox + g->cosA * xc1 - g->sinA * yc1);
     int cdpy = (int)((int)p->oy + g->sinA * xc1 + g->cosA * yc1);
-    /* field offsets from the write order of Kirchhoff.c:1499-1590, which is
-     * the header struct order of READ_SEGY.c:29-115 */
-    put_be32(out + 0, tracl);
-    put_be32(out + 20, cdp);
-    put_be32(out + 36, offset);
-    put_be32(out + 72, 1); /* sx, sy, gx, gy = htonl(1), Kirchhoff.c:1165-1168 */
-    put_be32(out + 76, 1);
-    put_be32(out + 80, 1);
-    put_be32(out + 84, 1);
-    put_be16(out + 114, (uint16_t)ns);
-    put_be16(out + 116, (uint16_t)(dt * 1000000));
-    put_be32(out + 180, cdpx);
-    put_be32(out + 184, cdpy);
-    put_be32(out + 188, iline);
-    put_be32(out + 192, cline);
-    put_be32(out + 236, 9999); /* Kirchhoff.c:1589-1590 */
+    item[H_TRACL] = tracl;
+    item[H_CDP] = cdp;
+    item[H_OFFSET] = offset;
+    item[H_SX] = item[H_SY] = item[H_GX] = item[H_GY] = 1; /* htonl(1), Kirchhoff.c:1165-1168 */
+    item[H_NS] = (uint16_t)ns;
+    item[H_DT] = (uint16_t)(dt * 1000000);
+    item[H_CDPX] = cdpx;
+    item[H_CDPY] = cdpy;
+    item[H_ILINE] = iline;
+    item[H_CLINE] = cline;
+    item[H_MESS0] = item[H_MESS1] = item[H_MESS2] = item[H_MESS3] = mess;
+    item[H_END] = 9999; /* Kirchhoff.c:1589-1590 */
+    emit_header(item, out);
+}
+
+void orc_segy_job_headers(const orc_params *p, const orc_grid *g, int ns, float dt, int job,
+                          uint8_t *out)
+{
+    /* Kirchhoff.c:1437-1598: it from nx*ny*nh*job + 1, one record per
+     * (xc, yc, ih); mess = 0 at 1224 and 9999 after the first record */
+    int it = g->nx * g->ny * p->nh * job + 1;
+    int32_t mess = 0;
+    for (int xc = 0; xc < g->nx; ++xc)
+        for (int yc = 0; yc < g->ny; ++yc)
+            for (int ih = 0; ih < p->nh; ++ih) {
+                orc_segy_trace_header(p, g, ns, dt, job, xc, yc, ih, it, mess, out);
+                mess = 9999;
+                out += 240;
+                ++it;
+            }
 }
 
 void orc_segy_file_header(const orc_params *p, const orc_grid *g, int ns, float dt,
@@ -917,4 +970,86 @@ void orc_eom_post(int ns, float dt, const float *lowpass, const float *rootrho, 
     }
     free(x); free(re); free(im); free(y);
     dft_free(&d);
+}
+
+void orc_eom_trace_header(const orc_eom_params *p, int ns, float dt, int job, int xc, int yc,
+                          int ih, int it, int32_t mess, uint8_t out[240])
+{
+    orc_eom_grid g;
+    orc_eom_grid_init(p, &g);
+    const int xs = job / p->jobs_y, ys = job % p->jobs_y;
+    int64_t item[H_COUNT] = {0};
+    /* EOM.c:1420-1435 */
+    const int offset = (int)ceil(p->oh + ih * p->dh);
+    const float cspX = (float)(g.start_x + (xc + xs * g.nx) * p->dcsp_x);
+    const float cspY = (float)(g.start_y + (yc + ys * g.ny) * p->dcsp_y);
+    const int xc1 = (int)(g.cosA * cspX - g.sinA * cspY);
+    const int yc1 = (int)(g.sinA * cspX + g.cosA * cspY);
+    const int csp = g.nx * g.ny * job + 1 + xc * g.ny + yc;   /* EOM.c:1346, 1536 */
+    item[H_TRACL] = it;
+    item[H_CDP] = csp;
+    item[H_OFFSET] = offset;
+    item[H_SX] = 13;   /* EOM.c:1120-1123 */
+    item[H_SY] = 14;
+    item[H_GX] = 15;
+    item[H_GY] = 16;
+    item[H_NS] = (uint16_t)ns;
+    item[H_DT] = (uint16_t)(dt * 1000000);
+    /* htonl(ox + xc1): a double converted to the uint32_t parameter (taken
+     * through int64 so a negative sum wraps as on x86-64) */
+    item[H_CDPX] = (uint32_t)(int64_t)(p->ox + xc1);
+    item[H_CDPY] = (uint32_t)(int64_t)(p->oy + yc1);
+    item[H_ILINE] = csp;
+    item[H_CLINE] = 1;   /* htonl(1.0) */
+    item[H_MESS0] = item[H_MESS1] = item[H_MESS2] = item[H_MESS3] = mess;
+    item[H_END] = 9999;  /* EOM.c:1527-1528 */
+    emit_header(item, out);
+}
+
+int64_t orc_eom_job_records(const orc_eom_params *p, int ns, float dt, int job,
+                            const float *post, uint8_t *out)
+{
+    orc_eom_grid g;
+    orc_eom_grid_init(p, &g);
+    if (job < 0 || job >= g.no_job) return -1;
+    const int64_t limit = (int64_t)g.Nx * g.Ny * p->nh + 1;   /* EOM.c:1417 */
+    int64_t it = (int64_t)g.nx * g.ny * p->nh * job + 1;       /* EOM.c:1347 */
+    int32_t mess = 0;                                          /* EOM.c:1179 */
+    int64_t written = 0;
+    const float *x = post;
+    for (int xc = 0; xc < g.nx; ++xc)
+        for (int yc = 0; yc < g.ny; ++yc)
+            for (int ih = 0; ih < p->nh; ++ih, ++it, x += ns) {
+                if (!(it < limit)) continue;
+                orc_eom_trace_header(p, ns, dt, job, xc, yc, ih, (int)it, mess, out);
+                mess = 9999;
+                out += 240;
+                /* data[0] = 0, data[t] = ntohf(out[t]) (EOM.c:1409-1412) */
+                put_be32(out, 0);
+                for (int t = 1; t < ns; ++t) {
+                    int32_t w;
+                    memcpy(&w, x + t, 4);
+                    put_be32(out + 4 * t, w);
+                }
+                out += 4 * (int64_t)ns;
+                ++written;
+            }
+    return written;
+}
+
+void orc_eom_file_header(int ns, float dt, uint8_t out[3600])
+{
+    /* EOM.c:479-480: the second sprintf overwrites the first; the rest of
+     * the 3200-byte buffer is left as zero here */
+    static const char msg[] = "written by G. O'Brien\n";
+    memset(out, 0, 3600);
+    memcpy(out, msg, sizeof msg - 1);
+    uint8_t *b = out + 3200;
+    put_be16(b + 16, (uint16_t)(dt * 1000000));   /* EOM.c:507-509 */
+    put_be16(b + 20, (uint16_t)ns);
+    put_be16(b + 22, (uint16_t)ns);
+    put_be16(b + 24, 5);                          /* IEEE, EOM.c:518-520 */
+    put_be16(b + 26, 1);
+    put_be16(b + 28, 1);
+    put_be16(b + 30, 1);
 }

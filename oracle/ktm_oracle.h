@@ -25,7 +25,8 @@
  *                         brute-force pure-Python loop on a tiny survey
  *   orc_poststack         direct DFT in exact rational/np.fft agreement,
  *                         linear-phase impulse response
- *   orc_segy_*            byte layout of READ_SEGY.c:29-115 (field order)
+ *   orc_segy_*            byte layout of READ_SEGY.c:29-115 (field order), the
+ *                         write sequence of Kirchhoff.c:1499-1590 word by word
  *   orc_eom_*             brute-force pure-Python loop, equivalent-offset closed
  *                         forms, linearity, fold count, np.fft for the filter
  *   orc_vgrad             FP64 central differences of J at every node, spline
@@ -134,10 +135,17 @@ void  orc_poststack_fp64(int ns, float dt, const float *anti_alias,
                          double *out);
 
 /* Kirchhoff.c:1437-1593: 240-byte header of output trace (xc,yc,ih) of a job,
- * tracl is the running trace number (Kirchhoff.c:1438). */
+ * tracl is the running trace number (Kirchhoff.c:1438) and mess the value the
+ * function-local `mess` holds when the record is written: 0 (Kirchhoff.c:1224)
+ * for the first record of a kirchhoff() call, 9999 (left by Kirchhoff.c:1589)
+ * for every later one.  It lands in bytes 205, 221, 225 and 233. */
 void  orc_segy_trace_header(const orc_params *p, const orc_grid *g, int ns, float dt,
-                            int job, int xc, int yc, int ih, int tracl,
+                            int job, int xc, int yc, int ih, int tracl, int32_t mess,
                             uint8_t out[240]);
+/* The headers of image<job>.su in write order (Kirchhoff.c:1437-1598), with
+ * the running tracl and the carried `mess`: out[nx*ny*nh][240]. */
+void  orc_segy_job_headers(const orc_params *p, const orc_grid *g, int ns, float dt,
+                           int job, uint8_t *out);
 /* Kirchhoff.c:729-814: 3200-byte text + 400-byte binary header; fix_count
  * writes the trace count the reference meant to write at byte 12. */
 void  orc_segy_file_header(const orc_params *p, const orc_grid *g, int ns, float dt,
@@ -202,6 +210,20 @@ int64_t orc_eom_job(const orc_eom_params *p, int ns, float dt, int job, const fl
  * lowpass FFT((csp/fold) dt t)) / ns, out[0] = 0. */
 void  orc_eom_post(int ns, float dt, const float *lowpass, const float *rootrho,
                    const float *csp, const float *fold, int64_t ntraces, float *out);
+/* EOM.c:1414-1500: 240-byte header of gather trace (xc,yc,ih) of a job; it is
+ * the running trace number, mess the carried value as in
+ * orc_segy_trace_header (EOM.c:1182 sets 0, EOM.c:1492 leaves 9999). */
+void  orc_eom_trace_header(const orc_eom_params *p, int ns, float dt, int job, int xc, int yc,
+                           int ih, int it, int32_t mess, uint8_t out[240]);
+/* image<job>.su of csp(job_id) (EOM.c:1364-1505): for (xc, yc, ih) in order,
+ * header and ns big-endian floats of post[nx][ny][nh][ns] (the post filter
+ * output), only while it < Nx*Ny*nh + 1 (EOM.c:1417).  out holds up to
+ * nx*ny*nh*(240 + 4 ns) bytes; returns the number of records written. */
+int64_t orc_eom_job_records(const orc_eom_params *p, int ns, float dt, int job,
+                            const float *post, uint8_t *out);
+/* EOM.c:450-537: ebdic_header.bin + binary_header.bin as 3600 bytes (the
+ * uninitialised short at binary byte 12 written as 0). */
+void  orc_eom_file_header(int ns, float dt, uint8_t out[3600]);
 
 #ifdef __cplusplus
 }

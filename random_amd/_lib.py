@@ -143,6 +143,15 @@ _PROTOS = {
     "ktm_eom_reset": (_i32, [_vp, _vp, _vp, _vp]),
     "ktm_eom_block": (_i32, [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp]),
     "ktm_eom_post": (_i32, [_vp, _vp, _vp, _vp, _vp]),
+    "ktm_eom_job_size": (_i64, [_vp]),
+    "ktm_eom_job_reset": (_i32, [_vp, _vp, _vp, _vp]),
+    "ktm_eom_job_block": (_i32, [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp]),
+    "ktm_eom_job_post": (_i32, [_vp, _vp, _vp, _vp, _vp]),
+    "ktm_eom_job_records": (_i32, [_vp, _vp, _i32, _vp, C.POINTER(_i64), _vp]),
+    "ktm_eom_globals_init": (_i32, [C.POINTER(KtmEomConfig), C.c_char_p, C.c_char_p]),
+    "csp": (None, [_i32]),
+    "ktm_eom_last_status": (_i32, []),
+    "ktm_eom_globals_free": (None, []),
     "ktm_segy_encode": (_i32, [_vp, _i64, _i32, _i32, _vp, _vp, _vp]),
     "ktm_image_records": (_i32, [_vp, _vp, _i32, _vp, _vp]),
     "ktm_device_alloc": (_i32, [C.POINTER(_vp), C.c_size_t]),

@@ -1508,6 +1508,8 @@ __global__ void k_image_headers(const ImgHdrArgs a, uint32_t *__restrict__ rec)
         case 46: v = __builtin_bswap32((uint32_t)(int)cy); break;               /* cdpy */
         case 47: v = __builtin_bswap32((uint32_t)(int)il); break;               /* iline */
         case 48: v = __builtin_bswap32((uint32_t)(int)xl); break;               /* xline */
+        case 51: case 55: case 56: case 58:     /* mess: 0 in the first record, */
+            v = r > 0 ? __builtin_bswap32(9999u) : 0u; break;   /* then 9999 (:1224, 1589) */
         case 59: v = __builtin_bswap32(9999u); break;                           /* :1589-1590 */
         default: break;
         }

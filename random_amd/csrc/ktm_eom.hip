@@ -23,6 +23,11 @@
  * k_eom_post: the post filter as a real even-kernel circular convolution in
  *   FP64 (the (1+i) factor of the sqrt-rho spectrum leaves the real part of a
  *   Hermitian product unchanged), g = IDFT(lowpass * rootrho) / ns.
+ * k_eom_records: the bytes of image<job>.su (EOM.c:1346-1537), one thread
+ *   per 32-bit word.
+ * The gathers are either the whole padded volume [nxp][nyp][nh][ns] or, for
+ * the ktm_eom_job_* entry points, one job's slab [nx][ny][nh][ns] (the
+ * reference's csp_data/fold of csp(), EOM.c:1190-1211).
  */
 #include <hip/hip_runtime.h>
 
@@ -272,12 +277,13 @@ struct EomArgs {
     int64_t ntr;
     const float *cspx, *cspy, *v2, *t2d;
     const double *sv;
-    float *csp, *fold;            /* [nxp][nyp][nh][ns] */
+    float *csp, *fold;            /* [nxp][nyp][nh][ns], or the job's [nx][ny][nh][ns] */
     unsigned long long *counter;
     float aperture2, dt;
     double dh;
-    int oh, ns, nh, nyp, nseg;
+    int oh, ns, nh, nseg;
     int rx0, ry0, nrx, nry;       /* CSP range of the launch */
+    int cx0, cy0, nyl;            /* gather array: CSP (cx0, cy0) first, nyl per X */
 };
 
 __global__ void __launch_bounds__(EOM_NT) k_eom_accum(const EomArgs a)
@@ -301,7 +307,7 @@ __global__ void __launch_bounds__(EOM_NT) k_eom_accum(const EomArgs a)
     const float den0 = tdt * (float)t * a.dt * a.v2[tc];          /* t*dt*t*dt*vel_t[t] */
     const double svt = a.sv[tc];
     const float t2dt = a.t2d[tc];
-    const int64_t col_base = ((int64_t)X * a.nyp + Y) * a.nh;
+    const int64_t col_base = ((int64_t)(X - a.cx0) * a.nyl + (Y - a.cy0)) * a.nh;
     uint64_t nev = 0;
     for (int64_t c0 = 0; c0 < a.ntr; c0 += EOM_NT) {
         /* ---- aperture gate of EOM.c:1268-1270 for 256 traces, in order ---- */
@@ -438,16 +444,18 @@ extern "C" int ktm_eom_reset(const ktm_eom_plan *p, float *csp, float *fold, voi
     return KTM_OK;
 }
 
-extern "C" int ktm_eom_block(const ktm_eom_plan *cp, const float *traces, const int32_t *coords,
-                             int64_t ntr, int job, float *csp, float *fold, uint64_t *counter,
-                             void *stream)
+/* Bin a block of traces into the gathers of every slab (job < 0) or one;
+ * local: the gathers hold the job's slab only, [nx][ny][nh][ns]. */
+static int eom_block(const ktm_eom_plan *cp, const float *traces, const int32_t *coords,
+                     int64_t ntr, int job, bool local, float *csp, float *fold, uint64_t *counter,
+                     void *stream)
 {
     if (!cp || ntr < 0) return KTM_ERR_ARG;
     if (!ntr) return KTM_OK;
     if (!traces || !coords || !csp || !fold) return KTM_ERR_ARG;
     ktm_eom_plan *p = const_cast<ktm_eom_plan *>(cp);
     const ktm_eom_setup *s = &p->s;
-    if (job >= s->no_job) return KTM_ERR_ARG;
+    if (job >= s->no_job || (local && job < 0)) return KTM_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
     EOM_TRY(hipSetDevice(p->device));
     if (ntr > p->geo_cap) {
@@ -467,13 +475,18 @@ extern "C" int ktm_eom_block(const ktm_eom_plan *cp, const float *traces, const 
     a.cspx = p->d_cspx; a.cspy = p->d_cspy; a.v2 = p->d_v2; a.t2d = p->d_t2d; a.sv = p->d_sv;
     a.csp = csp; a.fold = fold; a.counter = (unsigned long long *)counter;
     a.aperture2 = p->aperture2; a.dt = s->dt; a.dh = p->cfg.dh; a.oh = p->cfg.oh;
-    a.ns = s->ns; a.nh = s->nh; a.nyp = s->nyp;
+    a.ns = s->ns; a.nh = s->nh;
     a.nseg = (s->ns + EOM_NT - 1) / EOM_NT;
     if (job < 0) {
         a.rx0 = 0; a.ry0 = 0; a.nrx = s->nxp; a.nry = s->nyp;
     } else {   /* the job's slab, EOM.c:1261-1262 */
         a.rx0 = (job / p->cfg.jobs_y) * s->nx; a.ry0 = (job % p->cfg.jobs_y) * s->ny;
         a.nrx = s->nx; a.nry = s->ny;
+    }
+    if (local) {
+        a.cx0 = a.rx0; a.cy0 = a.ry0; a.nyl = s->ny;
+    } else {
+        a.cx0 = 0; a.cy0 = 0; a.nyl = s->nyp;
     }
     const int64_t ntiles = (int64_t)a.nrx * a.nry * a.nseg;
     if (ntiles > 0x7fffff00LL) return KTM_ERR_UNSUPPORTED;
@@ -483,11 +496,25 @@ extern "C" int ktm_eom_block(const ktm_eom_plan *cp, const float *traces, const 
     return KTM_OK;
 }
 
+extern "C" int ktm_eom_block(const ktm_eom_plan *p, const float *traces, const int32_t *coords,
+                             int64_t ntr, int job, float *csp, float *fold, uint64_t *counter,
+                             void *stream)
+{
+    return eom_block(p, traces, coords, ntr, job, false, csp, fold, counter, stream);
+}
+
+extern "C" int ktm_eom_job_block(const ktm_eom_plan *p, const float *traces,
+                                 const int32_t *coords, int64_t ntr, int job, float *csp,
+                                 float *fold, uint64_t *counter, void *stream)
+{
+    return eom_block(p, traces, coords, ntr, job, true, csp, fold, counter, stream);
+}
+
 typedef void (*eom_post_fn)(const float *, const float *, int64_t, int, float, const double *,
                             float *);
 
-extern "C" int ktm_eom_post(const ktm_eom_plan *p, const float *csp, const float *fold, float *out,
-                            void *stream)
+static int eom_post(const ktm_eom_plan *p, const float *csp, const float *fold, int64_t ntraces,
+                    float *out, void *stream)
 {
     if (!p || !csp || !fold || !out) return KTM_ERR_ARG;
     const int ns = p->s.ns;
@@ -498,9 +525,128 @@ extern "C" int ktm_eom_post(const ktm_eom_plan *p, const float *csp, const float
     eom_post_fn fn = slots <= 4 ? k_eom_post<4> : slots <= 8 ? k_eom_post<8> : k_eom_post<16>;
     EOM_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds));
-    const int64_t ntraces = (int64_t)p->s.nxp * p->s.nyp * p->s.nh;
     hipLaunchKernelGGL(fn, dim3((unsigned)((ntraces + EOM_PT - 1) / EOM_PT)), dim3(EOM_NT), lds,
                        (hipStream_t)stream, csp, fold, ntraces, ns, p->s.dt, p->d_g, out);
+    EOM_TRY(hipGetLastError());
+    return KTM_OK;
+}
+
+extern "C" int ktm_eom_post(const ktm_eom_plan *p, const float *csp, const float *fold, float *out,
+                            void *stream)
+{
+    if (!p) return KTM_ERR_ARG;
+    return eom_post(p, csp, fold, (int64_t)p->s.nxp * p->s.nyp * p->s.nh, out, stream);
+}
+
+extern "C" int64_t ktm_eom_job_size(const ktm_eom_plan *p)
+{
+    return p ? (int64_t)p->s.nx * p->s.ny * p->s.nh * p->s.ns : -1;
+}
+
+extern "C" int ktm_eom_job_reset(const ktm_eom_plan *p, float *csp, float *fold, void *stream)
+{
+    if (!p || !csp || !fold) return KTM_ERR_ARG;
+    hipLaunchKernelGGL(k_eom_reset, dim3(2048), dim3(256), 0, (hipStream_t)stream, csp, fold,
+                       ktm_eom_job_size(p));
+    EOM_TRY(hipGetLastError());
+    return KTM_OK;
+}
+
+extern "C" int ktm_eom_job_post(const ktm_eom_plan *p, const float *csp, const float *fold,
+                                float *out, void *stream)
+{
+    if (!p) return KTM_ERR_ARG;
+    return eom_post(p, csp, fold, (int64_t)p->s.nx * p->s.ny * p->s.nh, out, stream);
+}
+
+/* ------------------------------------------------------------------------ */
+/* image<job>.su records, EOM.c:1346-1537                                    */
+/* ------------------------------------------------------------------------ */
+struct EomRecArgs {
+    const float *cspx, *cspy;     /* plan tables, global CSP index */
+    int64_t nrec;                 /* records written (a prefix of the job's) */
+    int64_t it0;                  /* running number of the job's first record */
+    int nx, ny, nh, ns, X0, Y0, csp0;
+    double ox, oy, dh;
+    int oh;
+    float cosA, sinA;
+    uint32_t dt_us;
+};
+
+/* one thread per 32-bit word of the records: header word or sample */
+__global__ void k_eom_records(const EomRecArgs a, const float *__restrict__ post,
+                              uint32_t *__restrict__ rec)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t W = 60 + (int64_t)a.ns;
+    if (e >= a.nrec * W) return;
+    const int64_t r = e / W;
+    const int w = (int)(e - r * W);
+    uint32_t v = 0u;
+    if (w >= 60) {
+        /* data[0] = 0, data[t] = ntohf(out[t]) (EOM.c:1409-1412) */
+        const int t = w - 60;
+        v = t == 0 ? 0u : __builtin_bswap32(__float_as_uint(post[r * a.ns + t]));
+    } else {
+        const int ih = (int)(r % a.nh);
+        const int64_t c = r / a.nh;
+        const int xc = (int)(c / a.ny), yc = (int)(c % a.ny);
+        switch (w) {
+        case 0: v = __builtin_bswap32((uint32_t)(a.it0 + r)); break;                   /* tracl = it */
+        case 5: case 47:                                                                /* cdp, iline = csp */
+            v = __builtin_bswap32((uint32_t)(a.csp0 + xc * a.ny + yc)); break;
+        case 9: v = __builtin_bswap32((uint32_t)(int)ceil(a.oh + ih * a.dh)); break;   /* offset */
+        case 18: case 19: case 20: case 21:                                            /* sx..gy = 13..16 */
+            v = __builtin_bswap32((uint32_t)(13 + w - 18)); break;
+        case 28: v = __builtin_bswap32((uint32_t)(a.ns & 0xffff)); break;              /* ns, byte 114 */
+        case 29: v = __builtin_bswap32(a.dt_us << 16); break;                          /* dt, byte 116 */
+        case 45: case 46: {                                   /* cdpx, cdpy = htonl(o + (int)rotated) */
+            const float cX = a.cspx[a.X0 + xc], cY = a.cspy[a.Y0 + yc];
+            const int xc1 = (int)(a.cosA * cX - a.sinA * cY);
+            const int yc1 = (int)(a.sinA * cX + a.cosA * cY);
+            const double o = w == 45 ? a.ox + (double)xc1 : a.oy + (double)yc1;
+            v = __builtin_bswap32((uint32_t)(int64_t)o);
+            break;
+        }
+        case 48: v = __builtin_bswap32(1u); break;                                     /* cline */
+        case 51: case 55: case 56: case 58:   /* mess: 0 in the first record written, then 9999 */
+            v = r > 0 ? __builtin_bswap32(9999u) : 0u; break;
+        case 59: v = __builtin_bswap32(9999u); break;                                  /* EOM.c:1527-1528 */
+        default: break;
+        }
+    }
+    rec[e] = v;
+}
+
+extern "C" int ktm_eom_job_records(const ktm_eom_plan *p, const float *post, int job,
+                                   void *records, int64_t *nrec, void *stream)
+{
+    if (!p || !post || !records || !nrec || ((uintptr_t)records & 3u)) return KTM_ERR_ARG;
+    const ktm_eom_setup *s = &p->s;
+    if (job < 0 || job >= s->no_job) return KTM_ERR_ARG;
+    /* it runs from nx*ny*nh*job + 1 and a record is written while
+     * it < Nx*Ny*nh + 1 (EOM.c:1347, 1417): a prefix of the job's records */
+    const int64_t total = (int64_t)s->nx * s->ny * s->nh;
+    const int64_t it0 = total * job + 1;
+    int64_t n = (int64_t)s->Nx * s->Ny * s->nh + 1 - it0;
+    n = n < 0 ? 0 : n > total ? total : n;
+    *nrec = n;
+    if (!n) return KTM_OK;
+    EomRecArgs a;
+    a.cspx = p->d_cspx; a.cspy = p->d_cspy;
+    a.nrec = n; a.it0 = it0;
+    a.nx = s->nx; a.ny = s->ny; a.nh = s->nh; a.ns = s->ns;
+    a.X0 = (job / p->cfg.jobs_y) * s->nx; a.Y0 = (job % p->cfg.jobs_y) * s->ny;
+    a.csp0 = s->nx * s->ny * job + 1;                       /* EOM.c:1346 */
+    a.ox = p->cfg.ox; a.oy = p->cfg.oy; a.dh = p->cfg.dh; a.oh = p->cfg.oh;
+    a.cosA = s->cosA; a.sinA = s->sinA;
+    a.dt_us = (uint32_t)(uint16_t)(s->dt * 1000000);
+    const int64_t words = n * (60 + (int64_t)s->ns);
+    const int nt = 256;
+    if ((words + nt - 1) / nt > 0x7fffffffLL) return KTM_ERR_UNSUPPORTED;
+    EOM_TRY(hipSetDevice(p->device));
+    hipLaunchKernelGGL(k_eom_records, dim3((unsigned)((words + nt - 1) / nt)), dim3(nt), 0,
+                       (hipStream_t)stream, a, post, (uint32_t *)records);
     EOM_TRY(hipGetLastError());
     return KTM_OK;
 }

@@ -257,6 +257,12 @@ void ktm_segy_image_trace_header(const ktm_config *cfg, int ns, float dt, int jo
     ktm_put_be32(out + 184, (uint32_t)(int)cy);
     ktm_put_be32(out + 188, (uint32_t)(int)il);
     ktm_put_be32(out + 192, (uint32_t)(int)xl);
+    /* the local `mess` (zero at Kirchhoff.c:1224) is written at bytes 204,
+     * 220, 224 and 232 of every record and holds htonl(9999) from
+     * Kirchhoff.c:1589 on: 0 in the job file's first record, 9999 after */
+    static const int mess_at[4] = {204, 220, 224, 232};
+    if (xc != 0 || yc != 0 || ih != 0)
+        for (int k = 0; k < 4; ++k) ktm_put_be32(out + mess_at[k], 9999u);
     ktm_put_be32(out + 236, 9999u); /* Kirchhoff.c:1589-1590 */
 }
 
@@ -511,3 +517,151 @@ static int kirchhoff_job(int job_id)
 }
 
 void kirchhoff(int job_id) { G.status = kirchhoff_job(job_id); }
+
+/* ------------------------------------------------------------------------ */
+/* csp(job_id): the EOM.c signature                                          */
+/* ------------------------------------------------------------------------ */
+static struct {
+    int ready, status;
+    ktm_eom_config cfg;
+    char input_data[4096], image_dir[4096];
+    int ns, nt;
+    float dt;
+    ktm_eom_plan *plan;
+} E;
+
+int ktm_eom_last_status(void) { return E.status; }
+
+void ktm_eom_globals_free(void)
+{
+    ktm_eom_plan_destroy(E.plan);
+    memset(&E, 0, sizeof E);
+}
+
+static int write_whole(const char *path, const void *data, size_t n)
+{
+    FILE *f = fopen(path, "wb");
+    if (!f) return KTM_ERR_IO;
+    int ok = fwrite(data, 1, n, f) == n;
+    ok &= fclose(f) == 0;
+    return ok ? KTM_OK : KTM_ERR_IO;
+}
+
+int ktm_eom_globals_init(const ktm_eom_config *cfg, const char *input_data, const char *image_dir)
+{
+    ktm_eom_globals_free();
+    if (!cfg || !input_data || !image_dir) return E.status = KTM_ERR_ARG;
+    if (strlen(input_data) >= sizeof E.input_data || strlen(image_dir) >= sizeof E.image_dir)
+        return E.status = KTM_ERR_ARG;
+    E.cfg = *cfg;
+    strcpy(E.input_data, input_data);
+    strcpy(E.image_dir, image_dir);
+    /* count_size_file(), EOM.c:276 and 398-444 */
+    int st = ktm_segy_count(input_data, &E.ns, &E.dt, &E.nt);
+    if (st != KTM_OK) return E.status = st;
+    /* mkdir image_dir (EOM.c:330-337); an existing directory is fine */
+    if (mkdir(image_dir, 0777) != 0 && errno != EEXIST) return E.status = KTM_ERR_IO;
+    /* build_vel_model() with velocity_model 0 (EOM.c:869-878): velocityStart
+     * + (velocityEnd - velocityStart) * (i / ns) in int arithmetic, where
+     * i / ns is 0 for every sample (squared by the plan) */
+    float *vel = (float *)malloc(sizeof(float) * (size_t)E.ns);
+    if (!vel) return E.status = KTM_ERR_NOMEM;
+    for (int i = 0; i < E.ns; ++i)
+        vel[i] = (float)(cfg->velocity_start + (cfg->velocity_end - cfg->velocity_start) * (i / E.ns));
+    /* the CSP grid (EOM.c:279-320) and the filters (create_lowpass_filter,
+     * create_sqrho_filter, EOM.c:340-343) are set up by the plan */
+    st = ktm_eom_plan_create(&E.plan, cfg, E.ns, E.dt, vel);
+    free(vel);
+    if (st != KTM_OK) return E.status = st;
+    /* write_segy_ebdic_header(), EOM.c:350 and 450-537: the text of
+     * EOM.c:480 (the rest of the 3200 bytes, uninitialised there, as 0) and
+     * the binary header (the uninitialised short of EOM.c:503-505 as 0) */
+    uint8_t hdr[SEGY_FILE_HDR];
+    memset(hdr, 0, sizeof hdr);
+    static const char text[] = "written by G. O'Brien\n";
+    memcpy(hdr, text, sizeof text - 1);
+    uint8_t *bin = hdr + 3200;
+    ktm_put_be16(bin + 16, (uint16_t)(E.dt * 1000000));
+    ktm_put_be16(bin + 20, (uint16_t)E.ns);
+    ktm_put_be16(bin + 22, (uint16_t)E.ns);
+    ktm_put_be16(bin + 24, 5);
+    ktm_put_be16(bin + 26, 1);
+    ktm_put_be16(bin + 28, 1);
+    ktm_put_be16(bin + 30, 1);
+    char name[4200];
+    snprintf(name, sizeof name, "%s/ebdic_header.bin", image_dir);
+    st = write_whole(name, hdr, 3200);
+    snprintf(name, sizeof name, "%s/binary_header.bin", image_dir);
+    if (st == KTM_OK) st = write_whole(name, hdr + 3200, 400);
+    if (st != KTM_OK) return E.status = st;
+    E.ready = 1;
+    return E.status = KTM_OK;
+}
+
+static int eom_csp_job(int job_id)
+{
+    if (!E.ready) return KTM_ERR_ARG;
+    ktm_eom_setup s;
+    ktm_eom_plan_setup(E.plan, &s);
+    if (job_id < 0 || job_id >= s.no_job) return KTM_ERR_ARG;
+    const size_t ng = (size_t)ktm_eom_job_size(E.plan);
+    const int64_t blk = E.cfg.ntblock > 0 ? E.cfg.ntblock : 20000;
+    const int64_t nb = E.nt < blk ? E.nt : blk;
+    const int64_t rec = SEGY_TRACE_HDR + 4 * (int64_t)s.ns;
+    const size_t nrec_max = (size_t)s.nx * s.ny * s.nh;
+    float *d_csp = NULL, *d_fold = NULL, *d_out = NULL, *d_tr = NULL;
+    int32_t *d_co = NULL;
+    void *d_raw = NULL, *d_rec = NULL;
+    uint8_t *hbuf = NULL;
+    FILE *f = NULL;
+    /* csp_data = 0, fold = 1 for the job's slab (EOM.c:1192-1211) */
+    int st = ktm_device_alloc((void **)&d_csp, sizeof(float) * ng);
+    if (st == KTM_OK) st = ktm_device_alloc((void **)&d_fold, sizeof(float) * ng);
+    if (st == KTM_OK) st = ktm_eom_job_reset(E.plan, d_csp, d_fold, NULL);
+    /* the trace loop in blocks of ntblock (EOM.c:1221-1316): raw records
+     * read as stored, IBM-decoded and binned on the device */
+    if (st == KTM_OK && nb > 0) {
+        if (!(hbuf = (uint8_t *)malloc((size_t)(nb * rec)))) st = KTM_ERR_NOMEM;
+        if (st == KTM_OK && !(f = fopen(E.input_data, "rb"))) st = KTM_ERR_IO;
+        if (st == KTM_OK) st = ktm_device_alloc(&d_raw, (size_t)(nb * rec));
+        if (st == KTM_OK) st = ktm_device_alloc((void **)&d_tr, sizeof(float) * (size_t)(nb * s.ns));
+        if (st == KTM_OK) st = ktm_device_alloc((void **)&d_co, sizeof(int32_t) * (size_t)(4 * nb));
+    }
+    for (int64_t i0 = 0; i0 < E.nt && st == KTM_OK; i0 += blk) {
+        const int64_t n = E.nt - i0 < blk ? E.nt - i0 : blk;
+        st = read_records(f, i0, n, rec, hbuf);
+        if (st == KTM_OK) st = ktm_copy_to_device(d_raw, hbuf, (size_t)(n * rec));
+        if (st == KTM_OK) st = ktm_segy_decode(d_raw, n, s.ns, KTM_SEGY_IBM, d_tr, d_co, NULL);
+        if (st == KTM_OK)
+            st = ktm_eom_job_block(E.plan, d_tr, d_co, n, job_id, d_csp, d_fold, NULL, NULL);
+    }
+    if (f) fclose(f);
+    free(hbuf);
+    hbuf = NULL;
+    ktm_device_free(d_raw);
+    ktm_device_free(d_tr);
+    ktm_device_free(d_co);
+    /* scale, low-pass and sqrt-rho filter (EOM.c:1386-1413), then the
+     * records of image<job>.su (EOM.c:1346-1537), all on the device */
+    if (st == KTM_OK) st = ktm_device_alloc((void **)&d_out, sizeof(float) * ng);
+    if (st == KTM_OK) st = ktm_eom_job_post(E.plan, d_csp, d_fold, d_out, NULL);
+    ktm_device_free(d_csp);
+    ktm_device_free(d_fold);
+    int64_t nrec = 0;
+    if (st == KTM_OK) st = ktm_device_alloc(&d_rec, nrec_max * (size_t)rec);
+    if (st == KTM_OK) st = ktm_eom_job_records(E.plan, d_out, job_id, d_rec, &nrec, NULL);
+    const size_t bytes = (size_t)nrec * (size_t)rec;
+    if (st == KTM_OK && bytes && !(hbuf = (uint8_t *)malloc(bytes))) st = KTM_ERR_NOMEM;
+    if (st == KTM_OK && bytes) st = ktm_copy_to_host(hbuf, d_rec, bytes);
+    ktm_device_free(d_out);
+    ktm_device_free(d_rec);
+    if (st == KTM_OK) {
+        char name[4200];
+        snprintf(name, sizeof name, "%s/image%d.su", E.image_dir, job_id);
+        st = write_whole(name, hbuf, bytes);
+    }
+    free(hbuf);
+    return st;
+}
+
+void csp(int job_id) { E.status = eom_csp_job(job_id); }

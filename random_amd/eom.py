@@ -145,3 +145,51 @@ class EomPlan:
         check(lib().ktm_eom_post(self._h, csp.data_ptr(), fold.data_ptr(), out.data_ptr(),
                                  self._stream()), "ktm_eom_post")
         return out
+
+    # ---- one job's slab only, [nx][ny][nh][ns] (csp(job_id)'s arrays) ----
+    @property
+    def job_shape(self):
+        return (self.setup["nx"], self.setup["ny"], self.setup["nh"], self.ns)
+
+    def new_job_gathers(self, device="cuda"):
+        import torch
+        csp = torch.empty(self.job_shape, dtype=torch.float32, device=device)
+        fold = torch.empty_like(csp)
+        check(lib().ktm_eom_job_reset(self._h, csp.data_ptr(), fold.data_ptr(), self._stream()),
+              "ktm_eom_job_reset")
+        return csp, fold
+
+    def job_migrate(self, traces, coords, job: int, block: int | None = None):
+        """(csp, fold, evaluated samples) of one job's slab."""
+        import torch
+        block = block or self.cfg.ntblock
+        csp, fold = self.new_job_gathers(traces.device)
+        counter = torch.zeros(1, dtype=torch.int64, device=traces.device)
+        for i0 in range(0, traces.shape[0], block):
+            t, c = traces[i0:i0 + block], coords[i0:i0 + block]
+            assert t.dtype == torch.float32 and t.is_contiguous() and t.shape[1] == self.ns
+            assert c.dtype == torch.int32 and c.is_contiguous() and c.shape == (t.shape[0], 4)
+            check(lib().ktm_eom_job_block(self._h, t.data_ptr(), c.data_ptr(), t.shape[0], job,
+                                          csp.data_ptr(), fold.data_ptr(), counter.data_ptr(),
+                                          self._stream()), "ktm_eom_job_block")
+        return csp, fold, int(counter.item())
+
+    def job_post(self, csp, fold):
+        import torch
+        assert tuple(csp.shape) == self.job_shape and tuple(fold.shape) == self.job_shape
+        out = torch.empty_like(csp)
+        check(lib().ktm_eom_job_post(self._h, csp.data_ptr(), fold.data_ptr(), out.data_ptr(),
+                                     self._stream()), "ktm_eom_job_post")
+        return out
+
+    def job_records(self, post, job: int) -> bytes:
+        """The bytes of image<job>.su from the job's post-filtered gathers."""
+        import torch
+        assert tuple(post.shape) == self.job_shape and post.dtype == torch.float32
+        rec = 240 + 4 * self.ns
+        nx, ny, nh, _ = self.job_shape
+        buf = torch.empty(nx * ny * nh * rec, dtype=torch.uint8, device=post.device)
+        n = C.c_int64()
+        check(lib().ktm_eom_job_records(self._h, post.data_ptr(), job, buf.data_ptr(), C.byref(n),
+                                        self._stream()), "ktm_eom_job_records")
+        return buf[:n.value * rec].cpu().numpy().tobytes()

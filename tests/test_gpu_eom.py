@@ -2,7 +2,11 @@
 restatement of EOM.c csp(): fold and the evaluated-sample count bit for bit,
 the gathers to FP32 rounding (the double atan of the taper, EOM.c:1291, may
 differ from the C library's in its last bit), per job and for all jobs, on a
-line and on a rotated 3-D CSP grid, and the post filter (EOM.c:1386-1413)."""
+line and on a rotated 3-D CSP grid, the post filter (EOM.c:1386-1413), the
+job-local entry points, the image<job>.su records (EOM.c:1346-1537) and the
+csp(job_id) shim end to end."""
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -88,3 +92,108 @@ def test_eom_post_matches_oracle(cuda):
     e = rel_l2(out, ref)
     print(f"  post relL2 {e:.2e}")
     assert e <= 1e-6
+
+
+def test_eom_job_gathers_and_records(cuda):
+    """The job-local entry points (the arrays of csp(job_id)): gathers equal
+    to the full volume's slab bit for bit and to the oracle's job, the post
+    filter, and the image<job>.su bytes against the oracle's writer fed the
+    same post-filtered gathers.  excsp = 1480 leaves the last slab one CSP
+    short, so its records stop early (EOM.c:1417)."""
+    print("case: EOM job-local gathers, post and records", flush=True)
+    sv, cfg, tr = line_case()
+    cfg = cfg.replace(excsp=1480.0)
+    plan = EomPlan(cfg, sv.ns, sv.dt)
+    assert (plan.setup["Nx"], plan.setup["nx"]) == (119, 40)
+    d_tr, d_co = torch.from_numpy(tr).to(cuda), torch.from_numpy(sv.coords).to(cuda)
+    full, ffold, _ = plan.migrate(d_tr, d_co)
+    lp, rr = oracle.eom_filters(cfg, sv.ns, np.float32(sv.dt))
+    nx, rec = plan.setup["nx"], 240 + 4 * sv.ns
+    for job in range(plan.setup["no_job"]):
+        cj, fj, nj = plan.job_migrate(d_tr, d_co, job)
+        assert torch.equal(cj, full[job * nx:(job + 1) * nx])
+        assert torch.equal(fj, ffold[job * nx:(job + 1) * nx])
+        oc, of, on = oracle.eom_job(cfg, sv.ns, np.float32(sv.dt), job, tr, sv.coords,
+                                    cfg.velocity(sv.ns))
+        assert nj == on and np.array_equal(fj.cpu().numpy(), of)
+        assert rel_l2(cj.cpu().numpy(), oc) <= 1e-6
+        pj = plan.job_post(cj, fj)
+        ref = oracle.eom_post(cj.cpu().numpy(), fj.cpu().numpy(), np.float32(sv.dt), lp, rr)
+        assert rel_l2(pj.cpu().numpy(), ref) <= 1e-6
+        data = plan.job_records(pj, job)
+        want = oracle.eom_job_records(cfg, sv.ns, np.float32(sv.dt), job, pj.cpu().numpy())
+        print(f"  job {job}: {len(data) // rec} records")
+        assert len(data) == len(want) == (39 if job == 2 else 40) * cfg.nh * rec
+        assert data == want
+
+
+def test_eom_records_rotated_2d_jobs(cuda):
+    """Record bytes on a rotated 3-D CSP grid with 2 x 3 job slabs (cdpx/cdpy
+    from the rotated CSP position, cdp/iline numbering, the record-count
+    prefix rule) for random post-filtered gathers."""
+    print("case: EOM records, rotated 2 x 3 job grid", flush=True)
+    cfg = EomConfig(ox=4900.0, oy=5100.0, angle_e_to_n=12.0, oxcsp=5000.0, oycsp=5000.0,
+                    excsp=5400.0, eycsp=5300.0, scale_xy=1.0, dcsp_x=25.0, dcsp_y=30.0, dh=20.0,
+                    nh=5, aperture=500.0, jobs_x=2, jobs_y=3, ntblock=128,
+                    velocity_start=1800.0, velocity_end=1800.0)
+    ns, dt = 97, np.float32(0.002)
+    plan = EomPlan(cfg, ns, dt)
+    rng = np.random.default_rng(11)
+    counts = []
+    for job in range(plan.setup["no_job"]):
+        post = rng.standard_normal(plan.job_shape).astype(np.float32)
+        data = plan.job_records(torch.from_numpy(post).to(cuda), job)
+        assert data == oracle.eom_job_records(cfg, ns, dt, job, post)
+        counts.append(len(data) // (240 + 4 * ns))
+    print(f"  records per job {counts}")
+    nrec = plan.setup["nx"] * plan.setup["ny"] * cfg.nh
+    total = plan.setup["Nx"] * plan.setup["Ny"] * cfg.nh
+    assert counts == [min(nrec, max(0, total - nrec * j)) for j in range(6)]
+
+
+def test_eom_csp_shim_files(cuda, tmp_path):
+    """csp(job_id) end to end: an IBM SEG-Y line read in blocks of ntblock,
+    image<job>.su against the oracle's csp() restatement (headers byte for
+    byte, samples to FP32 rounding) and the header files."""
+    from test_gpu_shim import write_input
+    import ctypes as C
+    from random_amd import _lib
+    print("case: csp(job) shim, 3 jobs, IBM input", flush=True)
+    sv, cfg, tr = line_case()
+    cfg = cfg.replace(excsp=1480.0, ntblock=70)    # several blocks, the last one short
+    path = str(tmp_path / "in.sgy")
+    write_input(path, sv, tr)
+    raw = np.fromfile(path, np.uint8)[3600:].reshape(sv.ntr, 240 + 4 * sv.ns)[:, 240:]
+    tr_ibm = oracle.ibm_to_float(raw.ravel()).reshape(sv.ntr, sv.ns)
+    dt = np.float32(sv.dt)
+    lp, rr = oracle.eom_filters(cfg, sv.ns, dt)
+    L = _lib.lib()
+    c = cfg.to_c()
+    img_dir = str(tmp_path / "image_dir")
+    _lib.check(L.ktm_eom_globals_init(C.byref(c), path.encode(), img_dir.encode()),
+               "ktm_eom_globals_init")
+    rec = 240 + 4 * sv.ns
+    try:
+        g = oracle.eom_grid(cfg)
+        for job in range(g.no_job):
+            L.csp(job)
+            assert L.ktm_eom_last_status() == 0
+            data = open(os.path.join(img_dir, f"image{job}.su"), "rb").read()
+            oc, of, _ = oracle.eom_job(cfg, sv.ns, dt, job, tr_ibm, sv.coords, cfg.velocity(sv.ns))
+            want = oracle.eom_job_records(cfg, sv.ns, dt, job, oracle.eom_post(oc, of, dt, lp, rr))
+            assert len(data) == len(want)
+            got = np.frombuffer(data, np.uint8).reshape(-1, rec)
+            ref = np.frombuffer(want, np.uint8).reshape(-1, rec)
+            assert np.array_equal(got[:, :240], ref[:, :240])
+            x = got[:, 240:].copy().view(">f4").astype(np.float64)
+            y = ref[:, 240:].copy().view(">f4").astype(np.float64)
+            e = np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300)
+            print(f"  job {job}: {len(got)} records, samples relL2 {e:.2e}")
+            assert e <= 1e-5
+        fh = oracle.eom_file_header(sv.ns, dt)
+        assert open(os.path.join(img_dir, "ebdic_header.bin"), "rb").read() == fh[:3200]
+        assert open(os.path.join(img_dir, "binary_header.bin"), "rb").read() == fh[3200:]
+        L.csp(g.no_job)   # a job that does not exist is an error, not an exit
+        assert L.ktm_eom_last_status() == _lib.KTM_ERR_ARG
+    finally:
+        L.ktm_eom_globals_free()

@@ -165,6 +165,7 @@ def test_image_records_match_host_headers(cuda):
         _lib.check(L.ktm_image_records(plan._h, image.data_ptr(), job, rec.data_ptr(),
                                        torch.cuda.current_stream().cuda_stream), "ktm_image_records")
         got = rec.cpu().numpy().reshape(nx * ny * nh, 240 + 4 * ns)
+        assert np.array_equal(got[:, :240], oracle.segy_job_headers(cfg, ns, plan.dt, job))
         xs, ys = job // cfg.jobs_y, job % cfg.jobs_y
         r = 0
         for xc in range(nx):

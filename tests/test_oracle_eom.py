@@ -187,3 +187,95 @@ def test_post_filter_matches_numpy_fft():
     conv = np.array([[np.dot(np.roll(g[::-1], m + 1), x[j]) for m in range(ns)] for j in range(7)])
     conv[:, 0] = 0.0
     assert rel_l2(out, conv) < 1e-6
+
+
+# ---- image<job>.su (EOM.c:1346-1537) ----
+import struct  # noqa: E402
+
+# the 240-byte header in the field order of READ_SEGY.c:29-115 (EOM.c:140-230)
+TRACE_FMT = ">7i4h8i2h4i13h2H31h5i40x"
+FIELDS = (["tracl", "tracr", "fldr", "tracf", "ep", "cdp", "cdpt", "trid", "nvs", "nhs", "duse",
+           "offset", "gelev", "selev", "sdepth", "gdel", "sdel", "swdep", "gwdep", "scalel",
+           "scalco", "sx", "sy", "gx", "gy"] + [f"s88_{k}" for k in range(13)] + ["ns", "dt"] +
+          [f"s118_{k}" for k in range(31)] + ["cdpx", "cdpy", "iline", "cline", "shotpt"])
+
+
+def fields(h: bytes) -> dict:
+    return dict(zip(FIELDS, struct.unpack(TRACE_FMT, h)))
+
+
+def test_eom_header_known_values():
+    """The reference's own configuration (EOM.c:60-99), ns = 1500, dt = 4 ms:
+    CSP origin = survey origin, so cspX = X * 12.5, cspY = 0 and the rotation
+    by 0.3 degrees gives (int)(cos A cspX), (int)(sin A cspX)."""
+    cfg = EomConfig()
+    ns, dt = 1500, np.float32(0.004)
+    g = oracle.eom_grid(cfg)
+    assert (g.Nx, g.Ny, g.nx, g.ny, g.no_job) == (2407, 1, 67, 1, 36)
+    h = fields(oracle.eom_trace_header(cfg, ns, dt, 0, 0, 0, 0, 1, 0))
+    assert (h["tracl"], h["cdp"], h["iline"], h["cline"], h["offset"]) == (1, 1, 1, 1, 0)
+    assert (h["sx"], h["sy"], h["gx"], h["gy"]) == (13, 14, 15, 16)
+    assert (h["ns"], h["dt"]) == (1500, 4000)
+    assert (h["cdpx"], h["cdpy"]) == (789868, 265844)
+    # job 1, xc = 2, ih = 7: X = 69, cspX = 862.5
+    it = 67 * 1000 * 1 + 1 + 2 * 1000 + 7
+    hb = oracle.eom_trace_header(cfg, ns, dt, 1, 2, 0, 7, it, 9999)
+    h = fields(hb)
+    cosA = np.float32(math.cos(0.3 * 3.14159265359 / 180.0))
+    sinA = np.float32(math.sin(0.3 * 3.14159265359 / 180.0))
+    cx = np.float32(862.5)
+    assert h["tracl"] == it and h["cdp"] == h["iline"] == 67 + 1 + 2
+    assert h["offset"] == math.ceil(7 * 6.25) == 44
+    assert h["cdpx"] == 789868 + int(np.float32(cosA * cx)) == 789868 + 862
+    assert h["cdpy"] == 265844 + int(np.float32(sinA * cx)) == 265844 + 4
+    words = [struct.unpack(">i", hb[o:o + 4])[0] for o in (204, 220, 224, 232, 236)]
+    assert words == [9999] * 5
+    assert not any(hb[196:204]) and not any(hb[208:220]) and not any(hb[228:232])
+
+
+def test_eom_records_prefix_mess_and_samples():
+    """Records are written while it < Nx*Ny*nh + 1 (EOM.c:1417): the last
+    job of the reference's grid (2407 CSPs in 36 slabs of 67) writes 62 CSPs;
+    mess is 0 in the first record written and 9999 after (EOM.c:1179, 1527);
+    samples are the big-endian post values with the first set to 0."""
+    cfg = EomConfig(nh=3)
+    ns, dt = 8, np.float32(0.004)
+    g = oracle.eom_grid(cfg)
+    rng = np.random.default_rng(3)
+    post = rng.standard_normal((g.nx, g.ny, cfg.nh, ns)).astype(np.float32)
+    rec = 240 + 4 * ns
+    for job, ncsp in ((0, 67), (35, 2407 - 35 * 67)):
+        data = oracle.eom_job_records(cfg, ns, dt, job, post)
+        assert len(data) == ncsp * cfg.nh * rec
+        r = np.frombuffer(data, np.uint8).reshape(-1, rec)
+        f0, f1 = fields(r[0, :240].tobytes()), fields(r[1, :240].tobytes())
+        assert f0["tracl"] == 67 * 3 * job + 1 and f1["tracl"] == f0["tracl"] + 1
+        assert f0["cdp"] == 67 * job + 1 and fields(r[3, :240].tobytes())["cdp"] == 67 * job + 2
+        assert [f["offset"] for f in (f0, f1)] == [0, 7]
+        assert struct.unpack(">i", r[0, 204:208].tobytes())[0] == 0
+        assert all(struct.unpack(">i", r[k, o:o + 4].tobytes())[0] == 9999
+                   for k in (1, len(r) - 1) for o in (204, 220, 224, 232, 236))
+        x = r[:, 240:].copy().view(">f4").astype(np.float32).reshape(ncsp, 1, cfg.nh, ns)
+        assert not x[..., 0].any()
+        assert np.array_equal(x[..., 1:], post[:ncsp, :, :, 1:])
+    # a 2-D job grid: the rule counts records in job order, not CSPs inside
+    # the grid; with 2 x 3 slabs of 2 x 2 CSPs, jobs 0-3 write all their
+    # records (job 2's lie beyond Ny) and jobs 4-5 none
+    cfg2 = EomConfig(ox=0.0, oy=0.0, angle_e_to_n=0.0, oxcsp=0.0, oycsp=0.0, excsp=100.0,
+                     eycsp=50.0, dcsp_x=25.0, dcsp_y=12.5, nh=2, jobs_x=2, jobs_y=3)
+    g2 = oracle.eom_grid(cfg2)
+    assert (g2.Nx, g2.Ny, g2.nx, g2.ny) == (4, 4, 2, 2)
+    post2 = np.ones((2, 2, 2, ns), np.float32)
+    counts = [len(oracle.eom_job_records(cfg2, ns, dt, j, post2)) // rec for j in range(6)]
+    # Nx*Ny*nh = 32 records in all, 8 per job: jobs 0-3 write all, 4-5 none
+    assert counts == [8, 8, 8, 8, 0, 0]
+
+
+def test_eom_file_header():
+    h = oracle.eom_file_header(1500, np.float32(0.004))
+    assert h[:22] == b"written by G. O'Brien\n" and not any(h[22:3200])
+    b = h[3200:]
+    assert struct.unpack(">H", b[16:18])[0] == 4000
+    assert struct.unpack(">4H", b[20:28]) == (1500, 1500, 5, 1)
+    assert struct.unpack(">2H", b[28:32]) == (1, 1)
+    assert not any(b[:16]) and not any(b[32:])

@@ -29,7 +29,7 @@ def vp(a):
 def header_names():
     src = open(os.path.join(ROOT, "include", "ktm.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(ktm_\w+|kirchhoff)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(ktm_\w+|kirchhoff|csp)\s*\(", src)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -138,7 +138,8 @@ def test_image_trace_headers_match_oracle(cfg):
         out = np.zeros(240, np.uint8)
         L.ktm_segy_image_trace_header(C.byref(c), ns, dt, job, xc, yc, ih, vp(out))
         tracl = info["nx"] * info["ny"] * cfg.nh * job + 1 + (xc * info["ny"] + yc) * cfg.nh + ih
-        ref = oracle.segy_trace_header(cfg, ns, dt, job, xc, yc, ih, tracl)
+        mess = 0 if (xc, yc, ih) == (0, 0, 0) else 9999
+        ref = oracle.segy_trace_header(cfg, ns, dt, job, xc, yc, ih, tracl, mess)
         assert out.tobytes() == ref
         f = dict(zip(FIELDS, struct.unpack(TRACE_FMT, ref)))
         X = xc + (job // cfg.jobs_y) * info["nx"]
@@ -150,6 +151,41 @@ def test_image_trace_headers_match_oracle(cfg):
         assert f["offset"] == int(np.ceil(cfg.oh + ih * cfg.dh))
         assert (f["sx"], f["sy"], f["gx"], f["gy"]) == (1, 1, 1, 1)
         assert struct.unpack(">i", ref[236:240])[0] == 9999
+
+
+def test_job_headers_carry_mess():
+    """Kirchhoff.c writes its local `mess` (0 from :1224) at bytes 204, 220,
+    224 and 232 of every record and sets it to htonl(9999) at :1589 before
+    the last word: the first record of a job file has zeros there, every
+    later record 9999.  The oracle's write-sequence restatement of the whole
+    job against the host header writer, and the layout against the field
+    table of READ_SEGY.c:29-115."""
+    cfg = Config(jobs_x=3, jobs_y=2, start_xline=2500, end_xline=2510, nh=4, oh=150.0, dh=100.0)
+    ns, dt = 501, np.float32(0.002)
+    info, _ = setup_info(cfg, ns, dt)
+    L = _lib.lib()
+    c = cfg.to_c()
+    for job in (0, info["no_job"] - 1):
+        ref = oracle.segy_job_headers(cfg, ns, dt, job)
+        assert ref.shape == (info["nx"] * info["ny"] * cfg.nh, 240)
+        r = 0
+        for xc in range(info["nx"]):
+            for yc in range(info["ny"]):
+                for ih in range(cfg.nh):
+                    out = np.zeros(240, np.uint8)
+                    L.ktm_segy_image_trace_header(C.byref(c), ns, dt, job, xc, yc, ih, vp(out))
+                    assert np.array_equal(out, ref[r]), (job, xc, yc, ih)
+                    r += 1
+        tail = [struct.unpack(">i", bytes(ref[k, o:o + 4]))[0] for k in (0, 1, len(ref) - 1)
+                for o in (204, 220, 224, 232, 236)]
+        assert tail == [0, 0, 0, 0, 9999] + [9999] * 10
+        # shotpt and the 2-byte items around them stay zero
+        for k in (0, 1):
+            assert not ref[k, 196:204].any() and not ref[k, 208:220].any()
+            assert not ref[k, 228:232].any()
+        f = [dict(zip(FIELDS, struct.unpack(TRACE_FMT, bytes(ref[k])))) for k in range(2)]
+        assert f[1]["tracl"] == f[0]["tracl"] + 1 == info["nx"] * info["ny"] * cfg.nh * job + 2
+        assert f[0]["ns"] == ns and f[0]["dt"] == 2000 and f[1]["offset"] == int(np.ceil(150.0 + 100.0))
 
 
 @pytest.mark.parametrize("cfg", CONFIGS)
