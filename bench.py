@@ -76,6 +76,39 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 3}), flush=True)
 
 
+def measured_peaks(dev, reps=5):
+    """(HBM GB/s of a 1 GiB device copy, counting read + write; LDS read GB/s
+    of the ktm_probe_lds kernel at full occupancy), measured on this GPU."""
+    import torch
+    from random_amd import _lib
+    n = 1 << 28
+    src = torch.ones(n, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_gbs = reps * 2 * n * 4 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst
+    nblocks, iters = 8192, 2048       # 8 workgroups of 16 KB LDS per CU, 256 CUs x 4
+    out = torch.empty(nblocks * 256, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    L = _lib.lib()
+    _lib.check(L.ktm_probe_lds(out.data_ptr(), nblocks, iters, stream), "ktm_probe_lds")
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        _lib.check(L.ktm_probe_lds(out.data_ptr(), nblocks, iters, stream), "ktm_probe_lds")
+    e1.record()
+    torch.cuda.synchronize()
+    lds_gbs = reps * nblocks * 256 * iters * 64 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    return copy_gbs, lds_gbs
+
+
 def cpu_baseline(sv, traces_np, target_s, nthreads):
     """Oracle FP32 (the reference algorithm, strict IEEE, -O3) on the first S
     traces over the whole image, jobs = one slab per thread as the reference's
@@ -387,6 +420,19 @@ def main():
                      "kernel": "k_migrate", "bytes_per_contribution": BYTES_FP32,
                      "avg_launch_ms": round(float(np.mean(mig_ms)), 2)},
     }
+    if not args.profile:
+        copy_gbs, lds_gbs = measured_peaks(dev)
+        traffic_gbs = TRAFFIC / (np.mean(mig_ms) * 1e-3) / 1e9
+        # the kernel reads one 4-byte word of LDS per contribution (the three
+        # taps pre-combined into F(T) when the trace is staged)
+        lds_read = per_launch * 4 / (np.mean(mig_ms) * 1e-3) / 1e9
+        result["roofline"].update({
+            "hbm_copy_peak_measured": round(copy_gbs, 1),
+            "hbm_traffic_GB_per_s": round(traffic_gbs, 1),
+            "hbm_traffic_frac_of_copy_peak": round(traffic_gbs / copy_gbs, 4),
+            "lds": {"read_bytes_per_contribution": 4, "achieved": round(lds_read, 1),
+                    "peak_measured": round(lds_gbs, 1), "frac": round(lds_read / lds_gbs, 4),
+                    "unit": "GB/s"}})
     if not args.profile and args.leg_steps > 0:
         legs = {}
         for label, mode, dtype, accum, bpc in (

@@ -414,6 +414,10 @@ int ktm_synth_traces(const ktm_plan *plan, const ktm_trace_geom *geom, int64_t n
                      float *traces, void *stream);
 /* traces[i] += sigma * N(0,1), the normal deviate a function of (seed, i). */
 int ktm_synth_noise(float *traces, int64_t n, float sigma, uint64_t seed, void *stream);
+/* Measurement helper (no reference counterpart): an LDS read-bandwidth probe
+ * of nblocks workgroups of 256 lanes, each reading iters * 64 bytes per lane
+ * from LDS; out (device) receives nblocks * 256 words. */
+int ktm_probe_lds(uint32_t *out, int nblocks, int iters, void *stream);
 
 /* The float t such that, for every float x >= 0, x < t exactly when
  * (double)(float)sqrt((double)x) < A -- the form of the aperture and slab

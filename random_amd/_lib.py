@@ -143,6 +143,7 @@ _PROTOS = {
     "ktm_eom_reset": (_i32, [_vp, _vp, _vp, _vp]),
     "ktm_eom_block": (_i32, [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp]),
     "ktm_eom_post": (_i32, [_vp, _vp, _vp, _vp, _vp]),
+    "ktm_probe_lds": (_i32, [_vp, _i32, _i32, _vp]),
     "ktm_eom_job_size": (_i64, [_vp]),
     "ktm_eom_job_reset": (_i32, [_vp, _vp, _vp, _vp]),
     "ktm_eom_job_block": (_i32, [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp]),

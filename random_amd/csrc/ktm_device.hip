@@ -936,6 +936,38 @@ extern "C" int ktm_synth_noise(float *traces, int64_t n, float sigma, uint64_t s
     return KTM_OK;
 }
 
+/* LDS read bandwidth probe: every workgroup fills 16 KB of LDS, then each
+ * lane reads 16 bytes per step from a conflict-free rotating position,
+ * folded with xor so the VALU work per byte stays well below the LDS rate;
+ * one word per lane is written so the reads are not removed.  Reads
+ * nblocks * 256 * iters * 4 * 16 bytes of LDS. */
+__global__ void __launch_bounds__(256) k_probe_lds(uint32_t *__restrict__ out, int iters)
+{
+    __shared__ uint4 buf[4 * 256];
+    const int tid = threadIdx.x;
+    for (int k = 0; k < 4; ++k)
+        buf[k * 256 + tid] = make_uint4(tid * 4u + k, tid ^ 0x55u, blockIdx.x, k * 7u + 1u);
+    __syncthreads();
+    uint32_t a = 0u, b = 0u;
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint4 v = buf[k * 256 + ((tid + i) & 255)];
+            a ^= v.x ^ v.y;
+            b ^= v.z ^ v.w;
+        }
+    }
+    out[(int64_t)blockIdx.x * 256 + tid] = a ^ b;
+}
+
+extern "C" int ktm_probe_lds(uint32_t *out, int nblocks, int iters, void *stream)
+{
+    if (!out || nblocks < 1 || iters < 1) return KTM_ERR_ARG;
+    hipLaunchKernelGGL(k_probe_lds, dim3((unsigned)nblocks), dim3(256), 0, (hipStream_t)stream,
+                       out, iters);
+    HIP_TRY(hipGetLastError());
+    return KTM_OK;
+}
 
 /* ------------------------------------------------------------------------ */
 /* self-tests of the numerical assumptions of the fast traveltime path       */
