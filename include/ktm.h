@@ -176,8 +176,10 @@ int ktm_image_reset(const ktm_plan *plan, float *km, float *fold, float *comp,
  * km/fold/comp are [nxp][nyp][nh][ns]; comp is needed for KAHAN only.
  * Accumulation continues from the current contents, in trace order, so
  * successive blocks equal one pass over their concatenation.  If counter is
- * non-NULL (device uint64) the number of contributions (trace, column) x ns
- * is added to it. */
+ * non-NULL (device uint64) the number of contributions is added to it: ns per
+ * (trace, column) pair through the aperture gates of Kirchhoff.c:1338,1354,
+ * traces outside the offset classes included (the reference evaluates their
+ * loop body and drops the sum; the kernels skip them). */
 int ktm_migrate_block(const ktm_plan *plan, const void *prepared, int trace_dtype,
                       int prep_mode, const ktm_trace_geom *geom, int64_t ntr,
                       int accum, int job, float *km, float *fold, float *comp,

@@ -769,6 +769,31 @@ k_migrate(const MigArgs a)
     }
 }
 
+/* Contributions of the traces outside the offset classes [0, nh): the
+ * reference evaluates their loop body for every column in aperture and
+ * drops the sum (the off test of Kirchhoff.c:1387); k_migrate never stages
+ * them, so their (trace, column) pairs are counted here, with the gates of
+ * k_migrate, for the contribution count.  One thread per trace. */
+__global__ void k_count_outside(const MigArgs a)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.ntr) return;
+    const ktm_trace_geom g = a.geom[i];
+    if (g.off >= 0 && g.off < a.nh) return;
+    uint64_t n = 0;
+    for (int X = a.rx0; X < a.rx1; ++X) {
+        const float cx = a.colx[X] - g.cmpx;
+        if (!(cx * cx < a.ap_x2thr)) continue;   /* RN(cx^2 + cy^2) >= cx^2 */
+        for (int Y = a.ry0; Y < a.ry1; ++Y) {
+            const float cy = a.coly[Y] - g.cmpy;
+            const int job = (X / a.nx) * a.jobs_y + (Y / a.ny);
+            const float ex = a.slab_cx[job] - g.cmpx, ey = a.slab_cy[job] - g.cmpy;
+            n += (ex * ex + ey * ey < a.slab_x2thr[job] && cx * cx + cy * cy < a.ap_x2thr) ? 1u : 0u;
+        }
+    }
+    if (n) atomicAdd(a.counter, (unsigned long long)(n * (uint64_t)a.ns));
+}
+
 /* ------------------------------------------------------------------------ */
 /* image reset and post-stack                                                */
 /* ------------------------------------------------------------------------ */
@@ -1742,6 +1767,8 @@ extern "C" int ktm_migrate_block(const ktm_plan *p, const void *prepared, int dt
     const dim3 grid((unsigned)(per * 8), (unsigned)s->nh);
     HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(fn, grid, dim3(nthreads), lds, hs, a);
+    if (counter)
+        hipLaunchKernelGGL(k_count_outside, dim3((unsigned)((ntr + 255) / 256)), dim3(256), 0, hs, a);
     HIP_TRY(hipGetLastError());
     return KTM_OK;
 }
