@@ -7,7 +7,7 @@ import torch
 
 from random_amd import surveys
 from random_amd.ktm import Plan
-from random_amd.lsm import Operator, cgls, dot64, dot_test, migrate_op
+from random_amd.lsm import Operator, cgls, dot64, dot_test, migrate_op, time_gain
 
 pytestmark = pytest.mark.gpu
 
@@ -113,3 +113,13 @@ def test_lsm_configuration(cuda):
     assert all(b <= a * (1 + 1e-6) for a, b in zip(res, res[1:]))
     assert res[-1] < 0.9
     assert corr > 1.5 * corr0
+    # with the time-gain preconditioner (4 Hutchinson probes of diag(L^T L))
+    w = time_gain(op, plan.image_shape, nprobe=4, seed=5)
+    m_pc, hist_pc = cgls(op, d_obs, niter=20, precond=w)
+    res_pc = [h[0] / hist_pc[0][0] for h in hist_pc]
+    corr_pc = dot64(m_pc, m_true) / (dot64(m_pc, m_pc) * dot64(m_true, m_true)) ** 0.5
+    print("  preconditioned: relative residual " + " ".join(f"{r:.3f}" for r in res_pc[::4]) +
+          f" ... {res_pc[-1]:.3f}; correlation {corr_pc:.3f}")
+    assert all(b <= a * (1 + 1e-6) for a, b in zip(res_pc, res_pc[1:]))
+    assert res_pc[-1] < 0.65 * res[-1]
+    assert corr_pc > 2.0 * corr
