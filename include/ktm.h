@@ -109,8 +109,10 @@ enum { KTM_TRACE_F32 = 0, KTM_TRACE_BF16 = 1, KTM_TRACE_F16 = 2 };
 enum { KTM_ACCUM_PLAIN = 0, KTM_ACCUM_KAHAN = 1 };
 
 /* Device-resident setup: velocity model (squared, Kirchhoff.c:617), traveltime
- * tables and the image-column tables of one configuration. */
+ * tables and the image-column tables of one configuration.  Samples per
+ * trace: 4 <= ns <= KTM_MAX_NS (KTM_ERR_UNSUPPORTED otherwise). */
 typedef struct ktm_plan ktm_plan;
+#define KTM_MAX_NS 6144
 
 /* vel: host array [nvel_ilines][nvel_xlines][nsm] of RMS velocities as read
  * from the velocity SEG-Y (Kirchhoff.c:545).  Fails with KTM_ERR_VELOCITY if

@@ -54,7 +54,7 @@ struct ktm_plan {
     float *d_slab_x2thr;
     float *d_vel2;          /* [nvel_il][nvel_xl][nsm], squared like Kirchhoff.c:617 */
     float *d_w;             /* [ns+2] CLOSED weights */
-    double *d_hconv;        /* [ns]   post-stack kernel */
+    double *d_hconv2;       /* [2 ns] post-stack kernel, h2[m] = h[m mod ns] */
     float *d_stage;         /* FP32 filtered traces of a BF16/FP16 block */
     size_t stage_cap;       /* elements of d_stage */
     float *d_pix_v2, *d_pix_r;  /* per-pixel v^2 and RN(1/v^2), built for demigration */
@@ -807,24 +807,27 @@ __global__ void k_image_reset(float *km, float *fold, float *comp, int64_t n)
     }
 }
 
-/* Kirchhoff.c:1457-1479 for PS_TR image traces per workgroup.  The low-pass
+/* Kirchhoff.c:1457-1479 for TR image traces per workgroup.  The low-pass
  * multiplies the DFT by the even real spectrum anti_alias, which is the
  * circular convolution out[t] = sum_u h[(t-u) mod ns] d[u] with h the
- * (real) inverse DFT of anti_alias divided by ns; evaluated in FP64. */
-#define PS_TR 4
+ * (real) inverse DFT of anti_alias divided by ns; evaluated in FP64.
+ * h2[m] = h[m mod ns] for m < 2 ns is read from LDS, or (HG, long traces)
+ * straight from global memory where it is L1/L2 resident. */
 #define PS_NT 256
-template <int SLOTS>
+template <int SLOTS, int TR, bool HG>
 __global__ void __launch_bounds__(PS_NT)
 k_poststack(const float *__restrict__ km, const float *__restrict__ fold, int64_t ntraces,
-            int ns, float dt, const double *__restrict__ hconv, float *__restrict__ out)
+            int ns, float dt, const double *__restrict__ h2, float *__restrict__ out)
 {
     extern __shared__ double lds_d[];
-    double *hh = lds_d;              /* [2*ns]: hh[m] = h[m mod ns]      */
-    double *dd = lds_d + 2 * ns;     /* [ns][PS_TR] derivative traces    */
-    const int64_t tr0 = (int64_t)blockIdx.x * PS_TR;
-    for (int m = threadIdx.x; m < 2 * ns; m += PS_NT) hh[m] = hconv[m < ns ? m : m - ns];
-    for (int e = threadIdx.x; e < ns * PS_TR; e += PS_NT) {
-        const int u = e / PS_TR, b = e - u * PS_TR;
+    double *dd = lds_d;                          /* [ns][TR] derivative traces */
+    double *hl = lds_d + (int64_t)ns * TR;       /* [2*ns] when !HG            */
+    const double *hh = HG ? h2 : hl;
+    const int64_t tr0 = (int64_t)blockIdx.x * TR;
+    if (!HG)
+        for (int m = threadIdx.x; m < 2 * ns; m += PS_NT) hl[m] = h2[m];
+    for (int e = threadIdx.x; e < ns * TR; e += PS_NT) {
+        const int u = e / TR, b = e - u * TR;
         const int64_t tr = tr0 + b;
         double v = 0.0;
         if (tr < ntraces && u >= 1) {
@@ -832,24 +835,24 @@ k_poststack(const float *__restrict__ km, const float *__restrict__ fold, int64_
             /* Kirchhoff.c:1460: a float expression stored into a double */
             v = (double)(-dt * (float)u * (k[u] - k[u - 1]) * dt / fold[tr * ns + u]);
         }
-        dd[u * PS_TR + b] = v;
+        dd[u * TR + b] = v;
     }
     __syncthreads();
-    double acc[SLOTS][PS_TR];
+    double acc[SLOTS][TR];
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s)
 #pragma unroll
-        for (int b = 0; b < PS_TR; ++b) acc[s][b] = 0.0;
+        for (int b = 0; b < TR; ++b) acc[s][b] = 0.0;
     for (int u = 0; u < ns; ++u) {
-        double dv[PS_TR];
+        double dv[TR];
 #pragma unroll
-        for (int b = 0; b < PS_TR; ++b) dv[b] = dd[u * PS_TR + b];
+        for (int b = 0; b < TR; ++b) dv[b] = dd[u * TR + b];
 #pragma unroll
         for (int s = 0; s < SLOTS; ++s) {
             const int t = s * PS_NT + threadIdx.x;
             const double hv = t < ns ? hh[t - u + ns] : 0.0;
 #pragma unroll
-            for (int b = 0; b < PS_TR; ++b) acc[s][b] = __builtin_fma(hv, dv[b], acc[s][b]);
+            for (int b = 0; b < TR; ++b) acc[s][b] = __builtin_fma(hv, dv[b], acc[s][b]);
         }
     }
 #pragma unroll
@@ -857,7 +860,7 @@ k_poststack(const float *__restrict__ km, const float *__restrict__ fold, int64_
         const int t = s * PS_NT + threadIdx.x;
         if (t < ns) {
 #pragma unroll
-            for (int b = 0; b < PS_TR; ++b) {
+            for (int b = 0; b < TR; ++b) {
                 const int64_t tr = tr0 + b;
                 if (tr < ntraces) out[tr * ns + t] = t == 0 ? 0.0f : (float)acc[s][b];
             }
@@ -1181,39 +1184,41 @@ __global__ void __launch_bounds__(8 * KTM_WAVE) k_demigrate(const DemigArgs a)
 /* Transpose of the post-stack chain image = P0 H C D km of Kirchhoff.c:
  * 1457-1479 (D: first difference with d[0] = 0, C: -dt^2 t / fold[t],
  * H: the even circular low-pass, P0: out[0] = 0):  km_bar = D^T C H P0 img. */
-template <int SLOTS>
+template <int SLOTS, int TR, bool HG>
 __global__ void __launch_bounds__(PS_NT)
 k_poststack_adjoint(const float *__restrict__ img, const float *__restrict__ fold, int64_t ntraces,
-                    int ns, float dt, const double *__restrict__ hconv, float *__restrict__ out)
+                    int ns, float dt, const double *__restrict__ h2, float *__restrict__ out)
 {
     extern __shared__ double lds_d[];
-    double *hh = lds_d;              /* [2*ns] */
-    double *zz = lds_d + 2 * ns;     /* [ns][PS_TR] P0 img */
-    double *uu = zz + ns * PS_TR;    /* [ns+1][PS_TR] C H P0 img */
-    const int64_t tr0 = (int64_t)blockIdx.x * PS_TR;
-    for (int m = threadIdx.x; m < 2 * ns; m += PS_NT) hh[m] = hconv[m < ns ? m : m - ns];
-    for (int e = threadIdx.x; e < ns * PS_TR; e += PS_NT) {
-        const int u = e / PS_TR, b = e - u * PS_TR;
+    double *zz = lds_d;                          /* [ns][TR] P0 img           */
+    double *uu = zz + (int64_t)ns * TR;          /* [ns+1][TR] C H P0 img     */
+    double *hl = uu + (int64_t)(ns + 1) * TR;    /* [2*ns] when !HG           */
+    const double *hh = HG ? h2 : hl;
+    const int64_t tr0 = (int64_t)blockIdx.x * TR;
+    if (!HG)
+        for (int m = threadIdx.x; m < 2 * ns; m += PS_NT) hl[m] = h2[m];
+    for (int e = threadIdx.x; e < ns * TR; e += PS_NT) {
+        const int u = e / TR, b = e - u * TR;
         const int64_t tr = tr0 + b;
-        zz[u * PS_TR + b] = tr < ntraces && u >= 1 ? (double)img[tr * ns + u] : 0.0;
+        zz[u * TR + b] = tr < ntraces && u >= 1 ? (double)img[tr * ns + u] : 0.0;
     }
     __syncthreads();
-    double acc[SLOTS][PS_TR];
+    double acc[SLOTS][TR];
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s)
 #pragma unroll
-        for (int b = 0; b < PS_TR; ++b) acc[s][b] = 0.0;
+        for (int b = 0; b < TR; ++b) acc[s][b] = 0.0;
     for (int u = 0; u < ns; ++u) {
-        double zv[PS_TR];
+        double zv[TR];
 #pragma unroll
-        for (int b = 0; b < PS_TR; ++b) zv[b] = zz[u * PS_TR + b];
+        for (int b = 0; b < TR; ++b) zv[b] = zz[u * TR + b];
 #pragma unroll
         for (int s = 0; s < SLOTS; ++s) {
             const int t = s * PS_NT + threadIdx.x;
             /* H is symmetric: (H^T z)[t] = sum_u h[(u - t) mod ns] z[u] = sum_u h[(t - u) mod ns] z[u] */
             const double hv = t < ns ? hh[t - u + ns] : 0.0;
 #pragma unroll
-            for (int b = 0; b < PS_TR; ++b) acc[s][b] = __builtin_fma(hv, zv[b], acc[s][b]);
+            for (int b = 0; b < TR; ++b) acc[s][b] = __builtin_fma(hv, zv[b], acc[s][b]);
         }
     }
 #pragma unroll
@@ -1221,20 +1226,20 @@ k_poststack_adjoint(const float *__restrict__ img, const float *__restrict__ fol
         const int t = s * PS_NT + threadIdx.x;
         if (t < ns)
 #pragma unroll
-            for (int b = 0; b < PS_TR; ++b) {
+            for (int b = 0; b < TR; ++b) {
                 const int64_t tr = tr0 + b;
                 double c = 0.0;
                 if (tr < ntraces && t >= 1)
                     c = -(double)dt * (double)dt * (double)t / (double)fold[tr * ns + t];
-                uu[t * PS_TR + b] = c * acc[s][b];
+                uu[t * TR + b] = c * acc[s][b];
             }
     }
-    for (int b = threadIdx.x; b < PS_TR; b += PS_NT) uu[ns * PS_TR + b] = 0.0;
+    for (int b = threadIdx.x; b < TR; b += PS_NT) uu[ns * TR + b] = 0.0;
     __syncthreads();
-    for (int e = threadIdx.x; e < ns * PS_TR; e += PS_NT) {
-        const int t = e / PS_TR, b = e - t * PS_TR;
+    for (int e = threadIdx.x; e < ns * TR; e += PS_NT) {
+        const int t = e / TR, b = e - t * TR;
         const int64_t tr = tr0 + b;
-        if (tr < ntraces) out[tr * ns + t] = (float)(uu[t * PS_TR + b] - uu[(t + 1) * PS_TR + b]);
+        if (tr < ntraces) out[tr * ns + t] = (float)(uu[t * TR + b] - uu[(t + 1) * TR + b]);
     }
 }
 
@@ -1255,7 +1260,7 @@ extern "C" void ktm_plan_destroy(ktm_plan *p)
 {
     if (!p) return;
     void *bufs[] = {p->d_t2, p->d_t1, p->d_colx, p->d_coly, p->d_velx, p->d_vely,
-                    p->d_slab_cx, p->d_slab_cy, p->d_slab_x2thr, p->d_vel2, p->d_w, p->d_hconv,
+                    p->d_slab_cx, p->d_slab_cy, p->d_slab_x2thr, p->d_vel2, p->d_w, p->d_hconv2,
                     p->d_stage, p->d_pix_v2, p->d_pix_r};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -1268,7 +1273,7 @@ extern "C" int ktm_plan_create(ktm_plan **out, const ktm_config *cfg, int ns, fl
 {
     if (!out || !cfg || !vel) return KTM_ERR_ARG;
     *out = nullptr;
-    if (ns < 4 || ns > 3072) return KTM_ERR_UNSUPPORTED;
+    if (ns < 4 || ns > KTM_MAX_NS) return KTM_ERR_UNSUPPORTED;
     ktm_plan *p = (ktm_plan *)calloc(1, sizeof(ktm_plan));
     if (!p) return KTM_ERR_NOMEM;
     p->cfg = *cfg;
@@ -1303,7 +1308,13 @@ extern "C" int ktm_plan_create(ktm_plan **out, const ktm_config *cfg, int ns, fl
     if (st == KTM_OK) st = upload(&p->d_slab_cy, tb->slab_cy, (size_t)tb->s.no_job);
     if (st == KTM_OK) st = upload(&p->d_slab_x2thr, tb->slab_x2thr, (size_t)tb->s.no_job);
     if (st == KTM_OK) st = upload(&p->d_w, tb->w, (size_t)ns + 2);
-    if (st == KTM_OK) st = upload(&p->d_hconv, tb->hconv, (size_t)ns);
+    if (st == KTM_OK) {
+        double *h2 = (double *)malloc(sizeof(double) * 2 * (size_t)ns);
+        if (!h2) st = KTM_ERR_NOMEM;
+        for (int m = 0; st == KTM_OK && m < 2 * ns; ++m) h2[m] = tb->hconv[m < ns ? m : m - ns];
+        if (st == KTM_OK) st = upload(&p->d_hconv2, h2, 2 * (size_t)ns);
+        free(h2);
+    }
     if (st != KTM_OK) { ktm_plan_destroy(p); return st; }
     *out = p;
     return KTM_OK;
@@ -1665,6 +1676,15 @@ static mig_fn pick_accum(int accum, int *B)
     return accum == KTM_ACCUM_KAHAN ? k_migrate<K, W, 6, true> : k_migrate<K, W, 8, false>;
 }
 
+/* one column per workgroup (8 waves of K*64 samples) for long traces, with
+ * fewer traces staged per batch so the rows fit the LDS */
+template <int K, int BP, int BK>
+static mig_fn pick_long(int accum, int *B)
+{
+    *B = accum == KTM_ACCUM_PLAIN ? BP : BK;
+    return accum == KTM_ACCUM_KAHAN ? k_migrate<K, 8, BK, true> : k_migrate<K, 8, BP, false>;
+}
+
 /* W waves share one image column (each owns K*64 consecutive samples);
  * a workgroup is 8 waves, so it covers 8/W columns. */
 static mig_fn pick_migrate(int ns, int accum, int *W, int *K, int *B)
@@ -1675,8 +1695,10 @@ static mig_fn pick_migrate(int ns, int accum, int *W, int *K, int *B)
      * plain one; four waves of K = 8 per column measured 0.63x) */
     if (ns <= 1536) { *W = 2; *K = 12; return pick_accum<12, 2>(accum, B); }
     if (ns <= 2048) { *W = 4; *K = 8; return pick_accum<8, 4>(accum, B); }
-    *W = 4; *K = 12;
-    return pick_accum<12, 4>(accum, B);
+    if (ns <= 3072) { *W = 4; *K = 12; return pick_accum<12, 4>(accum, B); }
+    if (ns <= 4096) { *W = 8; *K = 8; return pick_long<8, 8, 6>(accum, B); }
+    *W = 8; *K = 12;
+    return pick_long<12, 5, 4>(accum, B);
 }
 
 /* FP32 filtered traces of a BF16/FP16 block, in a workspace the plan owns and
@@ -1773,23 +1795,72 @@ extern "C" int ktm_migrate_block(const ktm_plan *p, const void *prepared, int dt
     return KTM_OK;
 }
 
+/* post-stack kernel choice: traces per workgroup and where h lives, so the
+ * LDS (160 KB) holds the work arrays; SLOTS * 256 >= ns */
+typedef void (*ps_fn)(const float *, const float *, int64_t, int, float, const double *, float *);
+struct PsChoice {
+    ps_fn fn;
+    int tr;
+    size_t lds;
+};
+
+template <int SLOTS, int TR, bool HG>
+struct PsFwd {
+    static constexpr ps_fn fn = k_poststack<SLOTS, TR, HG>;
+};
+template <int SLOTS, int TR, bool HG>
+struct PsAdj {
+    static constexpr ps_fn fn = k_poststack_adjoint<SLOTS, TR, HG>;
+};
+
+template <template <int, int, bool> class Kern, int SLOTS>
+static bool ps_try(int ns, bool adjoint, PsChoice *c)
+{
+    /* doubles per traces-per-workgroup TR: forward ns*TR (+ 2 ns for h),
+     * adjoint (2 ns + 1)*TR (+ 2 ns) */
+    const size_t cap = 160 * 1024 / sizeof(double);
+    const size_t per = adjoint ? (size_t)(2 * ns + 1) : (size_t)ns, h = 2 * (size_t)ns;
+    if (4 * per + h <= cap) { *c = {Kern<SLOTS, 4, false>::fn, 4, sizeof(double) * (4 * per + h)}; return true; }
+    if (per + h <= cap) { *c = {Kern<SLOTS, 1, false>::fn, 1, sizeof(double) * (per + h)}; return true; }
+    if (per <= cap) { *c = {Kern<SLOTS, 1, true>::fn, 1, sizeof(double) * per}; return true; }
+    return false;
+}
+
+template <template <int, int, bool> class Kern>
+static int ps_pick(int ns, bool adjoint, PsChoice *c)
+{
+    const int slots = (ns + PS_NT - 1) / PS_NT;
+    bool ok = false;
+    if (slots <= 4) ok = ps_try<Kern, 4>(ns, adjoint, c);
+    else if (slots <= 8) ok = ps_try<Kern, 8>(ns, adjoint, c);
+    else if (slots <= 12) ok = ps_try<Kern, 12>(ns, adjoint, c);
+    else if (slots <= 16) ok = ps_try<Kern, 16>(ns, adjoint, c);
+    else if (slots <= 24) ok = ps_try<Kern, 24>(ns, adjoint, c);
+    return ok ? KTM_OK : KTM_ERR_UNSUPPORTED;
+}
+
+static int ps_launch(const ktm_plan *p, const PsChoice &c, const float *in, const float *fold,
+                     float *out, void *stream)
+{
+    const ktm_setup *s = &p->tb.s;
+    const int64_t ntraces = (int64_t)s->nxp * s->nyp * s->nh;
+    const int64_t nblk = (ntraces + c.tr - 1) / c.tr;
+    if (nblk > 0x7fffffffLL) return KTM_ERR_UNSUPPORTED;
+    HIP_TRY(hipFuncSetAttribute((const void *)c.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)c.lds));
+    hipLaunchKernelGGL(c.fn, dim3((unsigned)nblk), dim3(PS_NT), c.lds, (hipStream_t)stream, in,
+                       fold, ntraces, s->ns, s->dt, p->d_hconv2, out);
+    HIP_TRY(hipGetLastError());
+    return KTM_OK;
+}
+
 extern "C" int ktm_poststack(const ktm_plan *p, const float *km, const float *fold, float *out,
                              void *stream)
 {
     if (!p || !km || !fold || !out) return KTM_ERR_ARG;
-    const ktm_setup *s = &p->tb.s;
-    const int slots = (s->ns + PS_NT - 1) / PS_NT;
-    typedef void (*ps_fn)(const float *, const float *, int64_t, int, float, const double *, float *);
-    ps_fn fn = slots <= 4 ? k_poststack<4> : slots <= 8 ? k_poststack<8> : k_poststack<12>;
-    if (slots > 12) return KTM_ERR_UNSUPPORTED;
-    const int64_t ntraces = (int64_t)s->nxp * s->nyp * s->nh;
-    const size_t lds = sizeof(double) * (size_t)(2 * s->ns + s->ns * PS_TR);
-    const int64_t nblk = (ntraces + PS_TR - 1) / PS_TR;
-    HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(fn, dim3((unsigned)nblk), dim3(PS_NT), lds, (hipStream_t)stream,
-                       km, fold, ntraces, s->ns, s->dt, p->d_hconv, out);
-    HIP_TRY(hipGetLastError());
-    return KTM_OK;
+    PsChoice c;
+    const int st = ps_pick<PsFwd>(p->tb.s.ns, false, &c);
+    return st != KTM_OK ? st : ps_launch(p, c, km, fold, out, stream);
 }
 
 extern "C" int ktm_device_alloc(void **ptr, size_t bytes)
@@ -1864,7 +1935,9 @@ extern "C" int ktm_demigrate(const ktm_plan *p, const float *km, const ktm_trace
     else if (s->ns <= 1024) fn = k_demigrate<8, 2>;
     else if (s->ns <= 1536) fn = k_demigrate<12, 2>;
     else if (s->ns <= 2048) fn = k_demigrate<8, 4>;
-    else fn = k_demigrate<12, 4>;
+    else if (s->ns <= 3072) fn = k_demigrate<12, 4>;
+    else if (s->ns <= 4096) fn = k_demigrate<8, 8>;
+    else fn = k_demigrate<12, 8>;
     const size_t lds = sizeof(float) * (size_t)(s->ns + 4);
     for (int64_t i0 = 0; i0 < ntr; i0 += 1 << 30) {
         const int64_t n = ntr - i0 < (1 << 30) ? ntr - i0 : (1 << 30);
@@ -1882,18 +1955,7 @@ extern "C" int ktm_poststack_adjoint(const ktm_plan *p, const float *img, const 
                                      float *km_bar, void *stream)
 {
     if (!p || !img || !fold || !km_bar) return KTM_ERR_ARG;
-    const ktm_setup *s = &p->tb.s;
-    const int slots = (s->ns + PS_NT - 1) / PS_NT;
-    if (slots > 12) return KTM_ERR_UNSUPPORTED;
-    typedef void (*ps_fn)(const float *, const float *, int64_t, int, float, const double *, float *);
-    ps_fn fn = slots <= 4 ? k_poststack_adjoint<4> : slots <= 8 ? k_poststack_adjoint<8>
-                                                                 : k_poststack_adjoint<12>;
-    const int64_t ntraces = (int64_t)s->nxp * s->nyp * s->nh;
-    const size_t lds = sizeof(double) * (size_t)(2 * s->ns + s->ns * PS_TR + (s->ns + 1) * PS_TR);
-    const int64_t nblk = (ntraces + PS_TR - 1) / PS_TR;
-    HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(fn, dim3((unsigned)nblk), dim3(PS_NT), lds, (hipStream_t)stream, img, fold,
-                       ntraces, s->ns, s->dt, p->d_hconv, km_bar);
-    HIP_TRY(hipGetLastError());
-    return KTM_OK;
+    PsChoice c;
+    const int st = ps_pick<PsAdj>(p->tb.s.ns, true, &c);
+    return st != KTM_OK ? st : ps_launch(p, c, img, fold, km_bar, stream);
 }

@@ -1,9 +1,10 @@
 """Edge cases of the migration path on the GPU against the CPU oracle: every
 kernel variant (the sample-count boundaries where the per-column lane layout
-changes, from the smallest plan, ns = 4, to the largest, ns = 3072), empty
+changes, from the smallest plan, ns = 4, to the largest, ns = 6144), empty
 blocks, blocks with no trace in aperture, offset classes with traces outside
 [0, nh), a one-column image, the plan's size limits, and the adjoint of every
-variant."""
+variant, and the post-stack filter and its transpose for long traces (the
+traces-per-workgroup and kernel-in-LDS choices of ktm_poststack)."""
 import numpy as np
 import pytest
 import torch
@@ -17,8 +18,8 @@ pytestmark = pytest.mark.gpu
 
 # sample counts at and around the lane-layout boundaries of ktm_migrate_block
 # (K samples per lane x W waves per column: 8x1 <= 512 < 8x2 <= 1024 < 12x2
-# <= 1536 < 8x4 <= 2048 < 12x4 <= 3072)
-NS_CASES = [4, 63, 512, 513, 1024, 1025, 1536, 1537, 2048, 2049, 3072]
+# <= 1536 < 8x4 <= 2048 < 12x4 <= 3072 < 8x8 <= 4096 < 12x8 <= 6144)
+NS_CASES = [4, 63, 512, 513, 1024, 1025, 1536, 1537, 2048, 2049, 3072, 3073, 4096, 4097, 6144]
 
 
 def small_line(ns, **kw):
@@ -56,7 +57,7 @@ def test_every_kernel_variant_bitexact(cuda, ns):
     assert n > 0 and np.abs(km).max() > 0
 
 
-@pytest.mark.parametrize("ns", [4, 513, 1537, 2049, 3072])
+@pytest.mark.parametrize("ns", [4, 513, 1537, 2049, 3072, 4097, 6144])
 def test_adjoint_of_every_variant(cuda, ns):
     print(f"case: dot test at ns={ns}", flush=True)
     sv = small_line(ns)
@@ -119,7 +120,52 @@ def test_one_column_image(cuda):
 
 def test_plan_size_limits(cuda):
     sv = small_line(64)
-    for ns in (3, 3073):
+    for ns in (3, 6145):
         with pytest.raises(KtmError) as e:
             Plan(sv.cfg, ns, sv.dt, np.ones((sv.cfg.nvel_ilines, 1, ns), np.float32) * 2000.0)
         assert e.value.status == _lib.KTM_ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("ns", [1025, 2049, 4097, 6144])
+def test_poststack_long_traces(cuda, ns):
+    """Forward against the oracle's DFT path (TR = 4 up to 3413 samples, then
+    one trace per workgroup) and the dot test of the transpose (TR = 4 up to
+    2047, TR = 1 with h in LDS up to 5119, h from global memory beyond)."""
+    print(f"case: post-stack and transpose at ns={ns}", flush=True)
+    sv = small_line(ns, ncols=8, jobs_x=1)
+    plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
+    g = torch.Generator(device=cuda).manual_seed(ns + 1)
+    km = torch.randn(plan.image_shape, generator=g, device=cuda)
+    img = torch.randn(plan.image_shape, generator=g, device=cuda)
+    fold = torch.randint(1, 40, plan.image_shape, generator=g, device=cuda).float()
+    out = plan.poststack(km, fold).cpu().numpy()
+    tm, aa = oracle.alias_filter(sv.cfg, sv.ns, sv.dt)
+    ref = oracle.poststack(km.cpu().numpy()[:3], fold.cpu().numpy()[:3], sv.dt, aa)
+    r = float(np.linalg.norm(out[:3] - ref) / np.linalg.norm(ref))
+    lhs = float(torch.dot(plan.poststack(km, fold).double().ravel(), img.double().ravel()))
+    rhs = float(torch.dot(km.double().ravel(), plan.poststack_adjoint(img, fold).double().ravel()))
+    rel = abs(lhs - rhs) / abs(lhs)
+    print(f"  forward relL2 vs oracle {r:.2e}; transpose dot rel {rel:.2e}", flush=True)
+    assert r <= 1e-6
+    assert rel <= 1e-6
+
+
+@pytest.mark.parametrize("ns", [4097, 6144])
+def test_compensated_and_16bit_long_traces(cuda, ns):
+    """The Kahan and BF16 variants of the long-trace layouts against the FP64
+    oracle (same traveltime index, FP64 data path)."""
+    from random_amd import ACCUM_KAHAN, PREP_CLOSED, TRACE_BF16, TRACE_F32
+    print(f"case: Kahan FP32 and BF16 at ns={ns}", flush=True)
+    sv = small_line(ns)
+    tr = surveys.traces_numpy(sv)
+    km64, fold64, n64 = oracle.migrate(sv.cfg, sv.ns, sv.dt, tr, sv.coords, sv.vel,
+                                       mode=oracle.FP64_DATA)
+    plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
+    d_tr, d_co = torch.from_numpy(tr).to(cuda), torch.from_numpy(sv.coords).to(cuda)
+    for dtype, tol in ((TRACE_F32, 1e-6), (TRACE_BF16, 3e-3)):
+        km, fold, n = plan.migrate(d_tr, d_co, mode=PREP_CLOSED, dtype=dtype, accum=ACCUM_KAHAN,
+                                   block=25)
+        r = float(np.linalg.norm(km.cpu().numpy() - km64) / np.linalg.norm(km64))
+        print(f"  dtype {dtype}: relL2 vs FP64 {r:.2e}", flush=True)
+        assert n == n64 and np.array_equal(fold.cpu().numpy(), fold64.astype(np.float32))
+        assert r <= tol
