@@ -310,7 +310,7 @@ int ktm_eom_reset(const ktm_eom_plan *plan, float *csp, float *fold, void *strea
 int ktm_eom_block(const ktm_eom_plan *plan, const float *traces, const int32_t *coords,
                   int64_t ntr, int job, float *csp, float *fold, uint64_t *counter, void *stream);
 /* out = Re IFFT(rootrho (1+i) lowpass FFT((csp/fold) dt t)) / ns per gather
- * trace, out[..][0] = 0 (EOM.c:1386-1413), device. */
+ * trace, out[..][0] = 0 (EOM.c:1386-1413), device; ns <= 8192. */
 int ktm_eom_post(const ktm_eom_plan *plan, const float *csp, const float *fold, float *out,
                  void *stream);
 /* The same on one job's gathers only, [nx][ny][nh][ns] (csp_data and fold of

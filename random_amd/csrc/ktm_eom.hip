@@ -137,7 +137,7 @@ struct ktm_eom_plan {
     float *d_cspx, *d_cspy;   /* [nxp], [nyp] CSP coordinates, EOM.c:1261-1262 */
     float *d_v2, *d_t2d;      /* [ns] v^2 (EOM.c:877), sqrt(t dt) t dt (EOM.c:1196) */
     double *d_sv;             /* [ns] sqrt(v^2) in double (EOM.c:1290) */
-    double *d_g;              /* [ns] post-filter kernel */
+    double *d_g2;             /* [2 ns] post-filter kernel, g2[m] = g[m mod ns] */
     float4 *d_geo;            /* [cap] per-trace h2, cmpX, cmpY, talpha */
     int64_t geo_cap;
     float v2_0;               /* v^2 at t = 0 (talpha, EOM.c:1272) */
@@ -147,7 +147,7 @@ extern "C" int ktm_eom_plan_destroy(ktm_eom_plan *p)
 {
     if (!p) return KTM_OK;
     (void)hipSetDevice(p->device);
-    void *bufs[] = {p->d_cspx, p->d_cspy, p->d_v2, p->d_t2d, p->d_sv, p->d_g, p->d_geo};
+    void *bufs[] = {p->d_cspx, p->d_cspy, p->d_v2, p->d_t2d, p->d_sv, p->d_g2, p->d_geo};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free(p);
@@ -176,7 +176,7 @@ extern "C" int ktm_eom_plan_create(ktm_eom_plan **out, const ktm_eom_config *c, 
         v2 = (float *)malloc(sizeof(float) * (size_t)ns);
         t2d = (float *)malloc(sizeof(float) * (size_t)ns);
         sv = (double *)malloc(sizeof(double) * (size_t)ns);
-        g = (double *)malloc(sizeof(double) * (size_t)ns);
+        g = (double *)malloc(sizeof(double) * 2 * (size_t)ns);
         if (!cx || !cy || !v2 || !t2d || !sv || !g) st = KTM_ERR_NOMEM;
     }
     if (st == KTM_OK) {
@@ -200,7 +200,7 @@ extern "C" int ktm_eom_plan_create(ktm_eom_plan **out, const ktm_eom_config *c, 
                 km += m;
                 if (km >= ns) km -= ns;
             }
-            g[m] = (double)(acc / ns);
+            g[m] = g[m + ns] = (double)(acc / ns);
         }
     }
     if (st == KTM_OK && hipGetDevice(&p->device) != hipSuccess) st = KTM_ERR_DEVICE;
@@ -210,7 +210,7 @@ extern "C" int ktm_eom_plan_create(ktm_eom_plan **out, const ktm_eom_config *c, 
          hipMalloc(&p->d_v2, sizeof(float) * ns) != hipSuccess ||
          hipMalloc(&p->d_t2d, sizeof(float) * ns) != hipSuccess ||
          hipMalloc(&p->d_sv, sizeof(double) * ns) != hipSuccess ||
-         hipMalloc(&p->d_g, sizeof(double) * ns) != hipSuccess))
+         hipMalloc(&p->d_g2, sizeof(double) * 2 * ns) != hipSuccess))
         st = KTM_ERR_NOMEM;
     if (st == KTM_OK &&
         (hipMemcpy(p->d_cspx, cx, sizeof(float) * s->nxp, hipMemcpyHostToDevice) != hipSuccess ||
@@ -218,7 +218,7 @@ extern "C" int ktm_eom_plan_create(ktm_eom_plan **out, const ktm_eom_config *c, 
          hipMemcpy(p->d_v2, v2, sizeof(float) * ns, hipMemcpyHostToDevice) != hipSuccess ||
          hipMemcpy(p->d_t2d, t2d, sizeof(float) * ns, hipMemcpyHostToDevice) != hipSuccess ||
          hipMemcpy(p->d_sv, sv, sizeof(double) * ns, hipMemcpyHostToDevice) != hipSuccess ||
-         hipMemcpy(p->d_g, g, sizeof(double) * ns, hipMemcpyHostToDevice) != hipSuccess))
+         hipMemcpy(p->d_g2, g, sizeof(double) * 2 * ns, hipMemcpyHostToDevice) != hipSuccess))
         st = KTM_ERR_DEVICE;
     free(lp); free(rr); free(cx); free(cy); free(v2); free(t2d); free(sv); free(g);
     if (st != KTM_OK) {
@@ -380,43 +380,45 @@ __global__ void k_eom_reset(float *csp, float *fold, int64_t n)
     }
 }
 
-/* EOM.c:1386-1413 for EOM_PT gather traces per workgroup: x = (csp/fold) dt t
- * (float), out = g (*) x in FP64, out[0] = 0 */
-#define EOM_PT 4
-template <int SLOTS>
+/* EOM.c:1386-1413 for TR gather traces per workgroup: x = (csp/fold) dt t
+ * (float), out = g (*) x in FP64, out[0] = 0.  g2[m] = g[m mod ns] (m < 2 ns)
+ * is copied to LDS, or (HG, long traces) read from global memory. */
+template <int SLOTS, int TR, bool HG>
 __global__ void __launch_bounds__(EOM_NT)
 k_eom_post(const float *__restrict__ csp, const float *__restrict__ fold, int64_t ntraces, int ns,
-           float dt, const double *__restrict__ g, float *__restrict__ out)
+           float dt, const double *__restrict__ g2, float *__restrict__ out)
 {
     extern __shared__ double eom_lds[];
-    double *gg = eom_lds;              /* [2*ns]: gg[m] = g[m mod ns] */
-    double *xx = eom_lds + 2 * ns;     /* [ns][EOM_PT] */
-    const int64_t tr0 = (int64_t)blockIdx.x * EOM_PT;
-    for (int m = threadIdx.x; m < 2 * ns; m += EOM_NT) gg[m] = g[m < ns ? m : m - ns];
-    for (int e = threadIdx.x; e < ns * EOM_PT; e += EOM_NT) {
-        const int u = e / EOM_PT, b = e - u * EOM_PT;
+    double *xx = eom_lds;                          /* [ns][TR]              */
+    double *gl = eom_lds + (int64_t)ns * TR;       /* [2*ns] when !HG       */
+    const double *gg = HG ? g2 : gl;
+    const int64_t tr0 = (int64_t)blockIdx.x * TR;
+    if (!HG)
+        for (int m = threadIdx.x; m < 2 * ns; m += EOM_NT) gl[m] = g2[m];
+    for (int e = threadIdx.x; e < ns * TR; e += EOM_NT) {
+        const int u = e / TR, b = e - u * TR;
         const int64_t tr = tr0 + b;
         double v = 0.0;
         if (tr < ntraces && u >= 1)
             v = (double)((csp[tr * ns + u] / fold[tr * ns + u]) * dt * (float)u);   /* EOM.c:1391 */
-        xx[u * EOM_PT + b] = v;
+        xx[u * TR + b] = v;
     }
     __syncthreads();
-    double acc[SLOTS][EOM_PT];
+    double acc[SLOTS][TR];
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s)
 #pragma unroll
-        for (int b = 0; b < EOM_PT; ++b) acc[s][b] = 0.0;
+        for (int b = 0; b < TR; ++b) acc[s][b] = 0.0;
     for (int u = 0; u < ns; ++u) {
-        double xv[EOM_PT];
+        double xv[TR];
 #pragma unroll
-        for (int b = 0; b < EOM_PT; ++b) xv[b] = xx[u * EOM_PT + b];
+        for (int b = 0; b < TR; ++b) xv[b] = xx[u * TR + b];
 #pragma unroll
         for (int s = 0; s < SLOTS; ++s) {
             const int t = s * EOM_NT + threadIdx.x;
             const double gv = t < ns ? gg[t - u + ns] : 0.0;
 #pragma unroll
-            for (int b = 0; b < EOM_PT; ++b) acc[s][b] = __builtin_fma(gv, xv[b], acc[s][b]);
+            for (int b = 0; b < TR; ++b) acc[s][b] = __builtin_fma(gv, xv[b], acc[s][b]);
         }
     }
 #pragma unroll
@@ -424,7 +426,7 @@ k_eom_post(const float *__restrict__ csp, const float *__restrict__ fold, int64_
         const int t = s * EOM_NT + threadIdx.x;
         if (t < ns) {
 #pragma unroll
-            for (int b = 0; b < EOM_PT; ++b) {
+            for (int b = 0; b < TR; ++b) {
                 const int64_t tr = tr0 + b;
                 if (tr < ntraces) out[tr * ns + t] = t == 0 ? 0.0f : (float)acc[s][b];
             }
@@ -513,20 +515,42 @@ extern "C" int ktm_eom_job_block(const ktm_eom_plan *p, const float *traces,
 typedef void (*eom_post_fn)(const float *, const float *, int64_t, int, float, const double *,
                             float *);
 
+/* traces per workgroup and where g lives, so the FP64 arrays fit the LDS */
+template <int SLOTS>
+static bool eom_post_try(int ns, eom_post_fn *fn, int *tr, size_t *lds)
+{
+    const size_t cap = 160 * 1024 / sizeof(double), n = (size_t)ns;
+    if (6 * n <= cap) { *fn = k_eom_post<SLOTS, 4, false>; *tr = 4; *lds = 6 * n; }
+    else if (3 * n <= cap) { *fn = k_eom_post<SLOTS, 1, false>; *tr = 1; *lds = 3 * n; }
+    else if (n <= cap) { *fn = k_eom_post<SLOTS, 1, true>; *tr = 1; *lds = n; }
+    else return false;
+    *lds *= sizeof(double);
+    return true;
+}
+
 static int eom_post(const ktm_eom_plan *p, const float *csp, const float *fold, int64_t ntraces,
                     float *out, void *stream)
 {
     if (!p || !csp || !fold || !out) return KTM_ERR_ARG;
     const int ns = p->s.ns;
     const int slots = (ns + EOM_NT - 1) / EOM_NT;
-    if (slots > 16) return KTM_ERR_UNSUPPORTED;
-    const size_t lds = sizeof(double) * (size_t)(2 * ns + ns * EOM_PT);
-    if (lds > 160 * 1024) return KTM_ERR_UNSUPPORTED;
-    eom_post_fn fn = slots <= 4 ? k_eom_post<4> : slots <= 8 ? k_eom_post<8> : k_eom_post<16>;
+    eom_post_fn fn = nullptr;
+    int tr = 1;
+    size_t lds = 0;
+    bool ok = false;
+    if (slots <= 4) ok = eom_post_try<4>(ns, &fn, &tr, &lds);
+    else if (slots <= 8) ok = eom_post_try<8>(ns, &fn, &tr, &lds);
+    else if (slots <= 16) ok = eom_post_try<16>(ns, &fn, &tr, &lds);
+    else if (slots <= 24) ok = eom_post_try<24>(ns, &fn, &tr, &lds);
+    else if (slots <= 32) ok = eom_post_try<32>(ns, &fn, &tr, &lds);
+    if (!ok) return KTM_ERR_UNSUPPORTED;
+    const int64_t nblk = (ntraces + tr - 1) / tr;
+    if (nblk > 0x7fffffffLL) return KTM_ERR_UNSUPPORTED;
+    EOM_TRY(hipSetDevice(p->device));
     EOM_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds));
-    hipLaunchKernelGGL(fn, dim3((unsigned)((ntraces + EOM_PT - 1) / EOM_PT)), dim3(EOM_NT), lds,
-                       (hipStream_t)stream, csp, fold, ntraces, ns, p->s.dt, p->d_g, out);
+    hipLaunchKernelGGL(fn, dim3((unsigned)nblk), dim3(EOM_NT), lds, (hipStream_t)stream, csp, fold,
+                       ntraces, ns, p->s.dt, p->d_g2, out);
     EOM_TRY(hipGetLastError());
     return KTM_OK;
 }

@@ -197,3 +197,24 @@ def test_eom_csp_shim_files(cuda, tmp_path):
         assert L.ktm_eom_last_status() == _lib.KTM_ERR_ARG
     finally:
         L.ktm_eom_globals_free()
+
+
+@pytest.mark.parametrize("ns", [1001, 4097, 7000])
+def test_eom_post_long_traces(cuda, ns):
+    """The post filter with 4 gather traces per workgroup (short traces), one
+    per workgroup with g in LDS (4097) and with g read from global memory
+    (7000 samples), against the oracle's DFT path."""
+    print(f"case: EOM post filter at ns={ns}", flush=True)
+    cfg = EomConfig(ox=0.0, oy=0.0, angle_e_to_n=0.0, oxcsp=0.0, oycsp=0.0, excsp=50.0,
+                    eycsp=0.0, dcsp_x=12.5, dh=25.0, nh=3, jobs_x=1, jobs_y=1)
+    dt = np.float32(0.002)
+    plan = EomPlan(cfg, ns, dt)
+    g = torch.Generator(device=cuda).manual_seed(ns)
+    csp = torch.randn(plan.shape, generator=g, device=cuda)
+    fold = torch.randint(1, 30, plan.shape, generator=g, device=cuda).float()
+    out = plan.post(csp, fold).cpu().numpy()
+    lp, rr = oracle.eom_filters(cfg, ns, dt)
+    ref = oracle.eom_post(csp.cpu().numpy()[:2], fold.cpu().numpy()[:2], dt, lp, rr)
+    e = rel_l2(out[:2], ref)
+    print(f"  relL2 {e:.2e}", flush=True)
+    assert e <= 1e-6
