@@ -394,6 +394,15 @@ void kirchhoff(int job_id);
 int  ktm_last_status(void);
 void ktm_globals_free(void);
 
+/* combine_all_files() of Kirchhoff.c:821-874 (EOM.c:543-590 is the same):
+ * writes image_dir/combine_script -- cat of ebdic_header.bin,
+ * binary_header.bin and image0.su .. image<no_job-1>.su into output_data,
+ * then rm of the parts -- and makes it executable (a+x); it is not run. */
+int  ktm_combine_script(const char *image_dir, const char *output_data, int no_job);
+/* What that script's cat lines produce, done here: output_data = the two
+ * header files and every job's image in job order (the parts stay). */
+int  ktm_combine_files(const char *image_dir, const char *output_data, int no_job);
+
 /* csp(job_id), the entry point of EOM.c (EOM.c:1058), with main()'s setup
  * (EOM.c:274-345) done by ktm_eom_globals_init: count_size_file, the CSP
  * grid, the velocity of velocity_model 0 (EOM.c:869-878), the filters, mkdir

@@ -149,6 +149,8 @@ _PROTOS = {
     "ktm_eom_job_block": (_i32, [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp]),
     "ktm_eom_job_post": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "ktm_eom_job_records": (_i32, [_vp, _vp, _i32, _vp, C.POINTER(_i64), _vp]),
+    "ktm_combine_script": (_i32, [C.c_char_p, C.c_char_p, _i32]),
+    "ktm_combine_files": (_i32, [C.c_char_p, C.c_char_p, _i32]),
     "ktm_eom_globals_init": (_i32, [C.POINTER(KtmEomConfig), C.c_char_p, C.c_char_p]),
     "csp": (None, [_i32]),
     "ktm_eom_last_status": (_i32, []),

@@ -665,3 +665,68 @@ static int eom_csp_job(int job_id)
 }
 
 void csp(int job_id) { E.status = eom_csp_job(job_id); }
+
+/* ------------------------------------------------------------------------ */
+/* combine_all_files(): Kirchhoff.c:821-874, EOM.c:543-590                    */
+/* ------------------------------------------------------------------------ */
+int ktm_combine_script(const char *image_dir, const char *output_data, int no_job)
+{
+    if (!image_dir || !output_data || no_job < 0) return KTM_ERR_ARG;
+    char name[4200];
+    if (snprintf(name, sizeof name, "%s/combine_script", image_dir) >= (int)sizeof name)
+        return KTM_ERR_ARG;
+    FILE *f = fopen(name, "wb");
+    if (!f) return KTM_ERR_IO;
+    /* the lines of Kirchhoff.c:860-867: both header files, then every job's
+     * image in job order, then the parts removed (no final newline) */
+    int ok = fprintf(f, "cat ebdic_header.bin binary_header.bin > %s \n", output_data) > 0;
+    for (int i = 0; i < no_job && ok; ++i) ok = fprintf(f, "cat image%d.su >> %s\n", i, output_data) > 0;
+    ok = ok && fprintf(f, "rm image*.su\n") > 0;
+    ok = ok && fprintf(f, "rm ebdic_header.bin binary_header.bin ") > 0;
+    ok &= fclose(f) == 0;
+    if (!ok) return KTM_ERR_IO;
+    /* chmod a+x (Kirchhoff.c:869-870) */
+    struct stat sb;
+    if (stat(name, &sb) != 0 || chmod(name, (sb.st_mode & 07777) | 0111) != 0) return KTM_ERR_IO;
+    return KTM_OK;
+}
+
+static int append_file(FILE *out, const char *path, uint8_t *buf, size_t cap)
+{
+    FILE *in = fopen(path, "rb");
+    if (!in) return KTM_ERR_IO;
+    int st = KTM_OK;
+    for (;;) {
+        const size_t n = fread(buf, 1, cap, in);
+        if (n && fwrite(buf, 1, n, out) != n) { st = KTM_ERR_IO; break; }
+        if (n < cap) {
+            if (ferror(in)) st = KTM_ERR_IO;
+            break;
+        }
+    }
+    fclose(in);
+    return st;
+}
+
+int ktm_combine_files(const char *image_dir, const char *output_data, int no_job)
+{
+    if (!image_dir || !output_data || no_job < 0) return KTM_ERR_ARG;
+    const size_t cap = (size_t)1 << 22;
+    uint8_t *buf = (uint8_t *)malloc(cap);
+    if (!buf) return KTM_ERR_NOMEM;
+    FILE *out = fopen(output_data, "wb");
+    if (!out) { free(buf); return KTM_ERR_IO; }
+    char name[4200];
+    int st = KTM_OK;
+    snprintf(name, sizeof name, "%s/ebdic_header.bin", image_dir);
+    st = append_file(out, name, buf, cap);
+    snprintf(name, sizeof name, "%s/binary_header.bin", image_dir);
+    if (st == KTM_OK) st = append_file(out, name, buf, cap);
+    for (int i = 0; i < no_job && st == KTM_OK; ++i) {
+        snprintf(name, sizeof name, "%s/image%d.su", image_dir, i);
+        st = append_file(out, name, buf, cap);
+    }
+    if (fclose(out) != 0 && st == KTM_OK) st = KTM_ERR_IO;
+    free(buf);
+    return st;
+}

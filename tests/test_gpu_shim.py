@@ -65,6 +65,16 @@ def test_kirchhoff_job_files(cuda, tmp_path):
                 assert np.linalg.norm(got - ref) <= 1e-5 * max(np.linalg.norm(ref), 1e-30)
         fh = open(os.path.join(img_dir, "binary_header.bin"), "rb").read()
         assert fh == oracle.segy_file_header(sv.cfg, sv.ns, sv.dt)[3200:]
+        # main()'s combine step (Kirchhoff.c:355, 821-874): the script, and the
+        # file its cat lines make, which reads back as one SEG-Y of every job
+        _lib.check(L.ktm_combine_script(img_dir.encode(), b"out.sgy", g.no_job), "combine_script")
+        combined = str(tmp_path / "out.sgy")
+        _lib.check(L.ktm_combine_files(img_dir.encode(), combined.encode(), g.no_job), "combine")
+        ns_, dt_, nt_ = C.c_int(), C.c_float(), C.c_int()
+        _lib.check(L.ktm_segy_count(combined.encode(), C.byref(ns_), C.byref(dt_), C.byref(nt_)),
+                   "ktm_segy_count")
+        assert (ns_.value, nt_.value) == (sv.ns, g.no_job * g.nx * g.ny * sv.cfg.nh)
+        assert abs(dt_.value - sv.dt) < 1e-9
         L.kirchhoff(g.no_job)  # a job that does not exist is an error, not an exit
         assert L.ktm_last_status() == _lib.KTM_ERR_ARG
     finally:

@@ -239,3 +239,28 @@ def test_setup_errors():
     with pytest.raises(_lib.KtmError) as e:
         setup_info(Config(start_iline=1600, end_iline=1500), 1001, np.float32(0.004))
     assert e.value.status == _lib.KTM_ERR_ARG
+
+
+def test_combine_script_matches_reference_lines(tmp_path):
+    """combine_all_files() (Kirchhoff.c:852-870): the script text line by line
+    and its a+x mode; ktm_combine_files performs its cat lines."""
+    L = _lib.lib()
+    d = tmp_path / "image_dir"
+    d.mkdir()
+    _lib.check(L.ktm_combine_script(str(d).encode(), b"/data/out.sgy", 3), "ktm_combine_script")
+    text = (d / "combine_script").read_text()
+    assert text == ("cat ebdic_header.bin binary_header.bin > /data/out.sgy \n"
+                    "cat image0.su >> /data/out.sgy\n"
+                    "cat image1.su >> /data/out.sgy\n"
+                    "cat image2.su >> /data/out.sgy\n"
+                    "rm image*.su\n"
+                    "rm ebdic_header.bin binary_header.bin ")
+    assert os.stat(d / "combine_script").st_mode & 0o111 == 0o111
+    parts = [b"E" * 3200, b"B" * 400, b"job0" * 10, b"", b"job2" * 7]
+    for name, data in zip(["ebdic_header.bin", "binary_header.bin", "image0.su", "image1.su",
+                           "image2.su"], parts):
+        (d / name).write_bytes(data)
+    out = tmp_path / "all.sgy"
+    _lib.check(L.ktm_combine_files(str(d).encode(), str(out).encode(), 3), "ktm_combine_files")
+    assert out.read_bytes() == b"".join(parts)
+    assert L.ktm_combine_files(str(d).encode(), str(out).encode(), 4) == _lib.KTM_ERR_IO
