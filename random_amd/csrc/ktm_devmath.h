@@ -18,6 +18,17 @@ __device__ __forceinline__ float div_cr(float a, float b, float rb)
     return __builtin_fmaf(r, rb, q);
 }
 
+/* RN(a/b) in double from rb = RN(1/b): the same Markstein step (q = RN(a rb)
+ * is within one ulp of a/b, the FMA residual is exact, one correction rounds
+ * correctly), for quotients that are neither subnormal nor overflowing.  A
+ * zero b (rb infinite) gives NaN where a/b would give +-inf or NaN. */
+__device__ __forceinline__ double div_cr(double a, double b, double rb)
+{
+    const double q = a * rb;
+    const double r = __builtin_fma(-q, b, a);
+    return __builtin_fma(r, rb, q);
+}
+
 /* Correctly rounded double square root of 0 or a positive normal float:
  * reciprocal square-root seed and the Goldschmidt/Newton correction used by
  * the AMDGPU f64 sqrt expansion, without its denormal scaling (inputs are

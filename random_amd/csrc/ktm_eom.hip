@@ -16,8 +16,9 @@
  *   double promotions (EOM.c:1245-1272).
  * k_eom_accum: one workgroup = one CSP x one segment of 256 samples, one lane
  *   per sample.  The lane owns its sample of every offset bin of its CSP for
- *   the whole launch, so it adds into the gather in global memory without
- *   atomics and in trace order (the reference's order of additions per bin).
+ *   the whole launch, so it adds into the gather without atomics and in trace
+ *   order (the reference's order of additions per bin); the two bins of the
+ *   latest trace stay in registers until another trace needs their slot.
  *   Traces stream in chunks of 256: the aperture gate of each is evaluated
  *   lane-parallel, the passing ones are compacted in order through LDS.
  * k_eom_post: the post filter as a real even-kernel circular convolution in
@@ -306,9 +307,18 @@ __global__ void __launch_bounds__(EOM_NT) k_eom_accum(const EomArgs a)
     const float tdt = (float)t * a.dt;                            /* t*dt (float) */
     const float den0 = tdt * (float)t * a.dt * a.v2[tc];          /* t*dt*t*dt*vel_t[t] */
     const double svt = a.sv[tc];
+    /* reciprocals for the correctly rounded divisions below (div_cr): the
+     * quotients are those of the reference's double divisions */
+    const double rden0 = 1.0 / (double)den0;   /* +inf at t = 0: NaN offset, as 0-divides */
+    const double rsvt = 1.0 / svt, rdh = 1.0 / a.dh;
+    constexpr double R008 = 1.0 / 0.080, RPI = 1.0 / EOM_PI;
     const float t2dt = a.t2d[tc];
+    const float rt2d = 1.0f / t2dt;   /* t = 0 never bins: its offset is NaN */
     const int64_t col_base = ((int64_t)(X - a.cx0) * a.nyl + (Y - a.cy0)) * a.nh;
     uint64_t nev = 0;
+    /* the lane's sample of two offset bins, held in registers between traces */
+    int tagE = -1, tagO = -1;
+    float cE = 0.0f, fE = 0.0f, cO = 0.0f, fO = 0.0f;
     for (int64_t c0 = 0; c0 < a.ntr; c0 += EOM_NT) {
         /* ---- aperture gate of EOM.c:1268-1270 for 256 traces, in order ---- */
         const int64_t i = c0 + tid;
@@ -347,26 +357,65 @@ __global__ void __launch_bounds__(EOM_NT) k_eom_accum(const EomArgs a)
             nev += __popcll(__ballot(live));
             if (!live) continue;
             /* EOM.c:1285-1306 with the reference's promotions */
-            const float cross = (float)((4.0 * (double)x2j * (double)h2j) / (double)den0);
+            const float cross = (float)div_cr(4.0 * (double)x2j * (double)h2j, (double)den0, rden0);
             const float arg = x2j + h2j - cross;
             if (!(arg >= 0.0f)) continue;                 /* NaN offset: no contribution */
             const float eq = (float)sqrt_cr((double)arg);
-            const double qo = (double)(eq - (float)a.oh) / a.dh;
+            const double qo = div_cr((double)(eq - (float)a.oh), a.dh, rdh);
             const double fl = floor(qo);
             if (!(fl >= 0.0 && fl < (double)(a.nh - 1))) continue;
             const int off = (int)fl;
-            const float tcv = (float)(2.25 * (double)eq / svt);
-            const float s3 = (float)(0.5 + atan(200.0 * (double)(tdt - tcv) / 0.080) / EOM_PI);
+            const float tcv = (float)div_cr(2.25 * (double)eq, svt, rsvt);
+            const double z = div_cr(200.0 * (double)(tdt - tcv), 0.080, R008);
+            const float s3 = (float)(0.5 + div_cr(atan(z), EOM_PI, RPI));
             const float s2 = (float)(qo - (double)off);
             const float s1 = (float)(1.0 - (double)s2);
             const float x = a.traces[(c0 + sIdx[j]) * a.ns + t];
-            const int64_t e0 = (col_base + off) * a.ns + t;
-            a.csp[e0] = a.csp[e0] + s1 * s3 * x / t2dt;
-            a.fold[e0] = a.fold[e0] + 1.0f;
-            a.csp[e0 + a.ns] = a.csp[e0 + a.ns] + s2 * s3 * x / t2dt;
-            a.fold[e0 + a.ns] = a.fold[e0 + a.ns] + 1.0f;
+            const float v1 = div_cr(s1 * s3 * x, t2dt, rt2d), v2 = div_cr(s2 * s3 * x, t2dt, rt2d);
+            /* bins off and off+1 have opposite parity: the even one goes to
+             * slot E, the odd one to slot O; a slot that holds another bin
+             * is written back and refilled, so every bin lives in exactly
+             * one place and receives its adds in trace order */
+            const bool ev = (off & 1) == 0;
+            const int be = ev ? off : off + 1, bo = ev ? off + 1 : off;
+            if (tagE != be) {
+                if (tagE >= 0) {
+                    const int64_t w = (col_base + tagE) * a.ns + t;
+                    a.csp[w] = cE;
+                    a.fold[w] = fE;
+                }
+                const int64_t r = (col_base + be) * a.ns + t;
+                cE = a.csp[r];
+                fE = a.fold[r];
+                tagE = be;
+            }
+            if (tagO != bo) {
+                if (tagO >= 0) {
+                    const int64_t w = (col_base + tagO) * a.ns + t;
+                    a.csp[w] = cO;
+                    a.fold[w] = fO;
+                }
+                const int64_t r = (col_base + bo) * a.ns + t;
+                cO = a.csp[r];
+                fO = a.fold[r];
+                tagO = bo;
+            }
+            cE = cE + (ev ? v1 : v2);
+            fE = fE + 1.0f;
+            cO = cO + (ev ? v2 : v1);
+            fO = fO + 1.0f;
         }
         __syncthreads();
+    }
+    if (tagE >= 0) {
+        const int64_t w = (col_base + tagE) * a.ns + t;
+        a.csp[w] = cE;
+        a.fold[w] = fE;
+    }
+    if (tagO >= 0) {
+        const int64_t w = (col_base + tagO) * a.ns + t;
+        a.csp[w] = cO;
+        a.fold[w] = fO;
     }
     if (lane == 0 && a.counter) atomicAdd(a.counter, (unsigned long long)nev);
 }

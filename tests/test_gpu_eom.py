@@ -42,8 +42,9 @@ def test_eom_line_matches_oracle(cuda):
     csp, fold, n = plan.migrate(d_tr, d_co)
     c0, f0, n0 = oracle.eom_all(cfg, sv.ns, np.float32(sv.dt), tr, sv.coords, cfg.velocity(sv.ns))
     c, f = csp.cpu().numpy(), fold.cpu().numpy()
-    print(f"  evaluated {n} (oracle {n0}); gathers relL2 {rel_l2(c, c0):.2e}; "
-          f"fold cells differing {np.sum(f != f0)}")
+    same = float(np.mean(c.view(np.uint32) == c0.view(np.uint32)))
+    print(f"  evaluated {n} (oracle {n0}); gathers relL2 {rel_l2(c, c0):.2e}, bit-identical "
+          f"fraction {same:.8f}; fold cells differing {np.sum(f != f0)}")
     assert n == n0
     assert np.array_equal(f, f0)
     assert rel_l2(c, c0) <= 1e-6
@@ -75,7 +76,9 @@ def test_eom_rotated_grid_matches_oracle(cuda):
     csp, fold, n = plan.migrate(torch.from_numpy(tr).to(cuda), torch.from_numpy(sv.coords).to(cuda))
     c0, f0, n0 = oracle.eom_all(cfg, sv.ns, np.float32(sv.dt), tr, sv.coords, cfg.velocity(sv.ns))
     c, f = csp.cpu().numpy(), fold.cpu().numpy()
-    print(f"  evaluated {n} (oracle {n0}); gathers relL2 {rel_l2(c, c0):.2e}")
+    same = float(np.mean(c.view(np.uint32) == c0.view(np.uint32)))
+    print(f"  evaluated {n} (oracle {n0}); gathers relL2 {rel_l2(c, c0):.2e}, bit-identical "
+          f"fraction {same:.8f}")
     assert n == n0 and n > 0
     assert np.array_equal(f, f0)
     assert rel_l2(c, c0) <= 1e-6
