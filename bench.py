@@ -396,6 +396,9 @@ def main():
         torch.cuda.synchronize()
         total_ms = e_start.elapsed_time(e_end)
         n = int(counter.item())
+        # fold starts at 1 and counts the contributions that add into the image
+        # (T < ns, offset class in range: the "effective" ones)
+        run_mode.effective = int((fold.double() - 1.0).sum().item())
         del km, fold, comp
         return n / (total_ms * 1e-3) / 1e9, mig_ms, n, total_ms
 
@@ -414,7 +417,8 @@ def main():
                                "4 ms, image 400x400x1500 at 25 m, aperture 2500 m, trangeMax 8; "
                                "step = one 50,000-trace block (prep + migration)",
                    "traces": sv.ntr, "ns": sv.ns, "image": list(plan.image_shape[:2]) + [sv.ns],
-                   "block_traces": args.block, "contributions": n_contrib},
+                   "block_traces": args.block, "contributions": n_contrib,
+                   "effective_contributions": run_mode.effective},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC,
                      "kernel": "k_migrate", "bytes_per_contribution": BYTES_FP32,
