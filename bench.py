@@ -335,8 +335,8 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU decomposition is "
-                                   "described in DESIGN.md (image slabs need no collective)",
+        print(json.dumps({"error": "runs on one GPU only; across GPUs each would migrate its "
+                                   "own image slabs (job ids), which needs no collective",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import numpy as np

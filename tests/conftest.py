@@ -10,9 +10,9 @@ if ROOT not in sys.path:
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an AMD GPU (run on the MI355X box)")
-    # CPU-side builds only: the oracle and the library are compiled in-tree on
-    # the CPU when missing (the GPU box receives them prebuilt)
+    config.addinivalue_line("markers", "gpu: needs an AMD MI355X (gfx950) GPU")
+    # the oracle and the library are compiled in-tree when missing (a machine
+    # that only runs the tests receives them prebuilt)
     for d, out in (("oracle", "oracle/build/libktm_oracle.so"),
                    ("random_amd/csrc", "random_amd/libktm.so")):
         if not os.path.exists(os.path.join(ROOT, out)):
