@@ -1,0 +1,95 @@
+"""The reference programs' main() end to end on the GPU: kirchhoff_main reads
+an IBM SEG-Y survey and an IBM velocity SEG-Y, runs every job and combines the
+image into one SEG-Y (Kirchhoff.c:215-373); eom_main does the same for EOM.c
+(EOM.c:238-390).  The combined files are checked against the oracle's
+restatement: headers byte for byte, samples to FP32 rounding."""
+import dataclasses
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from random_amd import surveys
+from test_gpu_eom import line_case
+from test_gpu_shim import write_input
+from test_segy_abi import write_segy
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "random_amd", "_build")
+
+
+def settings(cfg):
+    return [f"{f.name}={getattr(cfg, f.name)!r}" for f in dataclasses.fields(cfg)]
+
+
+def compare(got: bytes, want: bytes, rec: int):
+    assert len(got) == len(want)
+    assert got[:3600] == want[:3600]
+    g = np.frombuffer(got[3600:], np.uint8).reshape(-1, rec)
+    w = np.frombuffer(want[3600:], np.uint8).reshape(-1, rec)
+    assert np.array_equal(g[:, :240], w[:, :240])
+    x = g[:, 240:].copy().view(">f4").astype(np.float64)
+    y = w[:, 240:].copy().view(">f4").astype(np.float64)
+    e = np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300)
+    print(f"  {len(g)} records, samples relL2 {e:.2e}", flush=True)
+    assert e <= 1e-5
+
+
+def test_kirchhoff_main_end_to_end(cuda, tmp_path):
+    print("case: kirchhoff_main, 2-D line, 3 jobs, IBM survey and velocity files", flush=True)
+    sv = surveys.line2d(nshots=6, ncols=96, ns=501, nscat=4, jobs_x=3)
+    sv.cfg.ntblock = 250
+    tr = surveys.traces_numpy(sv)
+    data, vel = str(tmp_path / "in.sgy"), str(tmp_path / "vel.sgy")
+    write_input(data, sv, tr)
+    nil = sv.cfg.nvel_ilines
+    ilxl = np.stack([np.arange(nil), np.zeros(nil, int)], 1)
+    write_segy(vel, sv.vel.reshape(nil, sv.ns), np.zeros((nil, 4), np.int32), ilxl, ieee=False)
+    img_dir, out = str(tmp_path / "image_dir"), str(tmp_path / "out.sgy")
+    r = subprocess.run([os.path.join(BIN, "kirchhoff_main"), data, vel, img_dir, out,
+                        *settings(sv.cfg)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    # the oracle on what the reference would have read: both files IBM-decoded
+    rec = 240 + 4 * sv.ns
+    raw = np.fromfile(data, np.uint8)[3600:].reshape(sv.ntr, rec)[:, 240:]
+    tr_ibm = oracle.ibm_to_float(raw.ravel()).reshape(sv.ntr, sv.ns)
+    rv = np.fromfile(vel, np.uint8)[3600:].reshape(nil, rec)[:, 240:]
+    vel_ibm = oracle.ibm_to_float(rv.ravel()).reshape(nil, 1, sv.ns)
+    km, fold, _ = oracle.migrate(sv.cfg, sv.ns, sv.dt, tr_ibm, sv.coords, vel_ibm)
+    _, aa = oracle.alias_filter(sv.cfg, sv.ns, sv.dt)
+    img = oracle.poststack(km, fold, sv.dt, aa)
+    g = oracle.grid(sv.cfg)
+    want = bytearray(oracle.segy_file_header(sv.cfg, sv.ns, sv.dt))
+    for job in range(g.no_job):
+        heads = oracle.segy_job_headers(sv.cfg, sv.ns, sv.dt, job)
+        for k in range(g.nx):
+            want += heads[k].tobytes() + img[job * g.nx + k, 0, 0].astype(">f4").tobytes()
+    compare(open(out, "rb").read(), bytes(want), rec)
+    script = open(os.path.join(img_dir, "combine_script")).read()
+    assert script.startswith(f"cat ebdic_header.bin binary_header.bin > {out} \n")
+
+
+def test_eom_main_end_to_end(cuda, tmp_path):
+    print("case: eom_main, 2-D line, 3 jobs, IBM survey", flush=True)
+    sv, cfg, tr = line_case()
+    cfg = cfg.replace(excsp=1480.0, ntblock=70)
+    data = str(tmp_path / "in.sgy")
+    write_input(data, sv, tr)
+    img_dir, out = str(tmp_path / "image_dir"), str(tmp_path / "out.sgy")
+    r = subprocess.run([os.path.join(BIN, "eom_main"), data, img_dir, out, *settings(cfg)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    rec = 240 + 4 * sv.ns
+    raw = np.fromfile(data, np.uint8)[3600:].reshape(sv.ntr, rec)[:, 240:]
+    tr_ibm = oracle.ibm_to_float(raw.ravel()).reshape(sv.ntr, sv.ns)
+    dt = np.float32(sv.dt)
+    lp, rr = oracle.eom_filters(cfg, sv.ns, dt)
+    want = bytearray(oracle.eom_file_header(sv.ns, dt))
+    for job in range(oracle.eom_grid(cfg).no_job):
+        c, f, _ = oracle.eom_job(cfg, sv.ns, dt, job, tr_ibm, sv.coords, cfg.velocity(sv.ns))
+        want += oracle.eom_job_records(cfg, sv.ns, dt, job, oracle.eom_post(c, f, dt, lp, rr))
+    compare(open(out, "rb").read(), bytes(want), rec)
