@@ -14,7 +14,8 @@ def pytest_configure(config):
     # the oracle and the library are compiled in-tree when missing (a machine
     # that only runs the tests receives them prebuilt)
     for d, out in (("oracle", "oracle/build/libktm_oracle.so"),
-                   ("random_amd/csrc", "random_amd/libktm.so")):
+                   ("random_amd/csrc", "random_amd/libktm.so"),
+                   ("random_amd/csrc", "random_amd/_build/eom_main")):
         if not os.path.exists(os.path.join(ROOT, out)):
             subprocess.run(["make", "-j8", "-C", os.path.join(ROOT, d)], check=True,
                            stdout=subprocess.DEVNULL)
