@@ -41,6 +41,7 @@
 #include "ktm_devmath.h"
 
 #define EOM_NT 256
+#define EOM_RUN 16     /* consecutive tiles per XCD run */
 #define EOM_WAVE 64
 #define EOM_PI 3.14159265359 /* EOM.c:124 */
 
@@ -292,10 +293,14 @@ __global__ void __launch_bounds__(EOM_NT) k_eom_accum(const EomArgs a)
     __shared__ float sX2[EOM_NT], sH2[EOM_NT];
     __shared__ int sTa[EOM_NT], sIdx[EOM_NT], sCnt[EOM_NT / EOM_WAVE];
     const int tid = threadIdx.x, lane = tid & (EOM_WAVE - 1), wv = tid / EOM_WAVE;
-    /* XCD-aware order: contiguous runs of (CSP, segment) tiles per XCD */
+    /* XCD-aware order: consecutive workgroup ids go to different XCDs; each
+     * XCD takes runs of EOM_RUN consecutive (CSP, segment) tiles (neighbouring
+     * CSPs share traces in its L2), and the runs are dealt round-robin so
+     * every XCD gets CSPs from the whole survey (the trace count per CSP
+     * falls towards the survey's edges) */
     const int ntiles = a.nrx * a.nry * a.nseg;
-    const int per = (ntiles + 7) >> 3;
-    const int tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int i = blockIdx.x >> 3;
+    const int tile = (((i / EOM_RUN) << 3) + (int)(blockIdx.x & 7)) * EOM_RUN + i % EOM_RUN;
     if (tile >= ntiles) return;
     const int seg = tile % a.nseg, col = tile / a.nseg;
     const int X = a.rx0 + col / a.nry, Y = a.ry0 + col % a.nry;
@@ -541,8 +546,9 @@ static int eom_block(const ktm_eom_plan *cp, const float *traces, const int32_t 
     }
     const int64_t ntiles = (int64_t)a.nrx * a.nry * a.nseg;
     if (ntiles > 0x7fffff00LL) return KTM_ERR_UNSUPPORTED;
-    const int per = (int)((ntiles + 7) / 8);
-    hipLaunchKernelGGL(k_eom_accum, dim3((unsigned)per * 8), dim3(EOM_NT), 0, st, a);
+    const int64_t span = 8 * EOM_RUN;   /* tiles per round of runs over the 8 XCDs */
+    const int64_t nblk = (ntiles + span - 1) / span * span;
+    hipLaunchKernelGGL(k_eom_accum, dim3((unsigned)nblk), dim3(EOM_NT), 0, st, a);
     EOM_TRY(hipGetLastError());
     return KTM_OK;
 }
