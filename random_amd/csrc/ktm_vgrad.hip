@@ -150,7 +150,7 @@ struct VgArgs {
     const int32_t *order, *ranges;
     const float *v, *colx;
     float coly, ap_x2thr, dt, rdt;
-    int ns, nh, nxp, chunk, nchunks;
+    int ns, nh, nxp, chunk, nchunks, run;
     float *cig;                    /* optional [nxp][nh][ns] */
     float *gv;                     /* optional [nxp][ns] */
     double *part;                  /* [nxp * nchunks] */
@@ -261,11 +261,13 @@ __global__ void __launch_bounds__(VG_NT) k_vgrad(const VgArgs a)
     double *sRed = vg_lds;                               /* [VG_NT / 64] */
     float *sI = reinterpret_cast<float *>(vg_lds + VG_NT / VG_WAVE);   /* [nh][VG_NT] */
     const int tid = threadIdx.x, lane = tid & (VG_WAVE - 1), wv = tid / VG_WAVE;
-    /* XCD-aware order: each XCD gets a contiguous run of (column, chunk)
-     * tiles, so neighbouring columns that share traces share its L2 */
+    /* XCD-aware order: consecutive workgroup ids go to different XCDs; each
+     * XCD takes runs of a.run consecutive (column, chunk) tiles (neighbouring
+     * columns share traces in its L2), dealt round-robin so that every XCD
+     * gets columns from the whole line (fewer traces reach its ends) */
     const int ntiles = a.nxp * a.nchunks;
-    const int per = (ntiles + 7) >> 3;
-    const int tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int wi = blockIdx.x >> 3;
+    const int tile = (((wi / a.run) << 3) + (int)(blockIdx.x & 7)) * a.run + wi % a.run;
     if (tile >= ntiles) return;
     const int X = tile / a.nchunks, ch = tile % a.nchunks;
     const int t = ch * a.chunk + tid;
@@ -520,11 +522,15 @@ extern "C" int ktm_vgrad_eval(ktm_vgrad *p, const float *V, double *J, double *d
     a.ns = p->ns; a.nh = p->nh; a.nxp = p->nxp; a.chunk = p->chunk; a.nchunks = p->nchunks;
     a.cig = cig; a.gv = dJdV ? p->d_gv : NULL; a.part = p->d_part;
     const int ntiles = p->nxp * p->nchunks;
-    const int per = (ntiles + 7) / 8;
+    /* runs of 16 columns, but at least 8 runs per XCD */
+    a.run = ntiles / 64 < 16 * p->nchunks ? ntiles / 64 : 16 * p->nchunks;
+    if (a.run < 1) a.run = 1;
+    const int64_t span = 8 * (int64_t)a.run;
+    const int64_t nblk = ((int64_t)ntiles + span - 1) / span * span;
     const size_t lds = vg_lds_bytes(p->nh);
     VG_TRY(hipFuncSetAttribute((const void *)k_vgrad, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));
-    hipLaunchKernelGGL(k_vgrad, dim3((unsigned)(per * 8)), dim3(VG_NT), lds, st, a);
+    hipLaunchKernelGGL(k_vgrad, dim3((unsigned)nblk), dim3(VG_NT), lds, st, a);
     VG_TRY(hipGetLastError());
     if (dJdV) {
         const int64_t n1 = (int64_t)p->nxp * p->nt_nodes;
