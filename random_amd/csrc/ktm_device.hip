@@ -1695,8 +1695,8 @@ static mig_fn pick_migrate(int ns, int accum, int *W, int *K, int *B)
 {
     if (ns <= 512) { *W = 1; *K = 8; return pick_accum<8, 1>(accum, B); }
     if (ns <= 1024) { *W = 2; *K = 8; return pick_accum<8, 2>(accum, B); }
-    /* (the Kahan K = 12 variant spills some registers, yet runs at 0.91x the
-     * plain one; four waves of K = 8 per column measured 0.63x) */
+    /* (the Kahan K = 12 variant spills some registers, yet runs at 0.87x the
+     * plain one; four waves of K = 6 per column measured 0.72x, of K = 8 0.63x) */
     if (ns <= 1536) { *W = 2; *K = 12; return pick_accum<12, 2>(accum, B); }
     if (ns <= 2048) { *W = 4; *K = 8; return pick_accum<8, 4>(accum, B); }
     if (ns <= 3072) { *W = 4; *K = 12; return pick_accum<12, 4>(accum, B); }
