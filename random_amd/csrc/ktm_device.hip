@@ -222,17 +222,22 @@ k_prep_ref(const TraceSrc src, int64_t ntr, int ns, int64_t row, float dt,
     const float last = load_sample<SRC>(src, i, ns - 1);
     float q = 0.0f, p = 0.0f;
     float pm1 = 0.0f, pm2 = 0.0f, pm3 = 0.0f, pm4 = 0.0f;
-    for (int t0 = 0; t0 < ns + 2; t0 += PREP_TS) {
-        const int t = t0 + sub;
-        /* unconditional loads at clamped (in-bounds) positions, so that all
-         * 32 are in flight before the first wait; out-of-range values are
-         * replaced afterwards */
-        const int tc = t < ns ? t : ns - 1;
-        const double den = t < ns && t > 0 ? 4.0 * (double)t2[tc] : 1.0;
-        uint32_t wv[KTM_WAVE / 2];
+    /* unconditional loads at clamped (in-bounds) positions, so that all 32
+     * are in flight before the first wait; out-of-range values are replaced
+     * after the decode.  The next tile's words are loaded as soon as this
+     * tile's are decoded, so their latency overlaps the recurrence. */
+    uint32_t wv[KTM_WAVE / 2];
+    float t2v;
+    {
+        const int tc = sub < ns ? sub : ns - 1;
 #pragma unroll
         for (int m = 0; m < KTM_WAVE / 2; ++m)
             wv[m] = load_word<SRC>(src, tr0 + min(2 * m + half, ntl - 1), tc);
+        t2v = t2[tc];
+    }
+    for (int t0 = 0; t0 < ns + 2; t0 += PREP_TS) {
+        const int t = t0 + sub;
+        const double den = t < ns && t > 0 ? 4.0 * (double)t2v : 1.0;
         float xv[KTM_WAVE / 2];
 #pragma unroll
         for (int m = 0; m < KTM_WAVE / 2; ++m)
@@ -240,6 +245,14 @@ k_prep_ref(const TraceSrc src, int64_t ntr, int ns, int64_t row, float dt,
 #pragma unroll
         for (int m = 0; m < KTM_WAVE / 2; ++m)
             s_inc[2 * m + half][sub] = t == 0 ? (double)xv[m] : (double)xv[m] / den;
+        if (t0 + PREP_TS < ns + 2) {
+            const int tn = t0 + PREP_TS + sub;
+            const int tc = tn < ns ? tn : ns - 1;
+#pragma unroll
+            for (int m = 0; m < KTM_WAVE / 2; ++m)
+                wv[m] = load_word<SRC>(src, tr0 + min(2 * m + half, ntl - 1), tc);
+            t2v = t2[tc];
+        }
         __syncthreads();
         const int s_end = min(PREP_TS, ns + 2 - t0);
         if (t0 >= 4 && t0 + PREP_TS <= ns) {
