@@ -93,3 +93,41 @@ def test_eom_main_end_to_end(cuda, tmp_path):
         c, f, _ = oracle.eom_job(cfg, sv.ns, dt, job, tr_ibm, sv.coords, cfg.velocity(sv.ns))
         want += oracle.eom_job_records(cfg, sv.ns, dt, job, oracle.eom_post(c, f, dt, lp, rr))
     compare(open(out, "rb").read(), bytes(want), rec)
+
+
+def test_kirchhoff_main_two_ranks(cuda, tmp_path):
+    """The round-robin job decomposition of Kirchhoff.c:347-350 as two
+    processes on the one GPU (rank/size in place of MPI): each writes its own
+    jobs' image files, rank 0 the combine script; the files equal those of one
+    process byte for byte."""
+    import ctypes as C
+    from random_amd import _lib
+    print("case: kirchhoff_main as 2 ranks vs 1, 3 jobs", flush=True)
+    sv = surveys.line2d(nshots=6, ncols=96, ns=501, nscat=4, jobs_x=3)
+    sv.cfg.ntblock = 250
+    tr = surveys.traces_numpy(sv)
+    data, vel = str(tmp_path / "in.sgy"), str(tmp_path / "vel.sgy")
+    write_input(data, sv, tr)
+    nil = sv.cfg.nvel_ilines
+    ilxl = np.stack([np.arange(nil), np.zeros(nil, int)], 1)
+    write_segy(vel, sv.vel.reshape(nil, sv.ns), np.zeros((nil, 4), np.int32), ilxl, ieee=False)
+    exe = os.path.join(BIN, "kirchhoff_main")
+    one = str(tmp_path / "one")
+    r = subprocess.run([exe, data, vel, one, str(tmp_path / "one.sgy"), *settings(sv.cfg)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    two = str(tmp_path / "two")
+    procs = [subprocess.Popen([exe, data, vel, two, str(tmp_path / "two.sgy"), f"rank={k}",
+                               "size=2", *settings(sv.cfg)], stderr=subprocess.PIPE, text=True)
+             for k in range(2)]
+    for p in procs:
+        _, err = p.communicate(timeout=300)
+        assert p.returncode == 0, err
+    for job in range(3):
+        a = open(os.path.join(one, f"image{job}.su"), "rb").read()
+        b = open(os.path.join(two, f"image{job}.su"), "rb").read()
+        assert a == b, job
+    assert os.path.exists(os.path.join(two, "combine_script"))
+    _lib.check(_lib.lib().ktm_combine_files(two.encode(), str(tmp_path / "two.sgy").encode(), 3),
+               "ktm_combine_files")
+    assert open(tmp_path / "two.sgy", "rb").read() == open(tmp_path / "one.sgy", "rb").read()
