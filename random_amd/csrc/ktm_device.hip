@@ -625,17 +625,16 @@ k_migrate(const MigArgs a)
 
 
     /* v^2 of this column for the exact path lives in LDS (t2 is read from
-     * the table there); the fast path keeps rs = RN(1/(v^2 dt^2)), tq =
-     * t^2/4 and, for the exact path, RN(1/v^2) in registers */
+     * the table there); the fast path (fast_qs: radicands pre-scaled by
+     * 1/dt^2) keeps rs = RN(1/(v^2 dt^2)) and tq = t^2/4 in registers.  The
+     * plain variants keep RN(1/v^2) for the exact path in registers too; the
+     * Kahan variants, whose compensation takes K registers, divide for it on
+     * the exact path. */
     float *sV2c = sV2 + cw * (W * K * KTM_WAVE);
     const int64_t pix = (((int64_t)Xc * a.nyp + Yc) * a.nh + h) * ns;
     const int64_t vbase = ((int64_t)a.velx[Xc] * a.nvel_xl + a.vely[Yc]) * a.nsm;
-    /* The plain variants use fast_qs (radicands pre-scaled by 1/dt^2: rs
-     * and tq in registers, t2 from the table on the exact path); the Kahan
-     * variants, with K more registers for the compensation, keep the
-     * fast_q form (1/v^2 and t2 in registers, a multiply by 1/dt). */
-    constexpr bool SCALED = !KAHAN;
-    float r[K], rs[K], tq[K], acc[K], cmp[K];
+    constexpr bool RREG = !KAHAN;           /* RN(1/v^2) per slot in registers */
+    float r[RREG ? K : 1], rs[K], tq[K], acc[K], cmp[K];
     int cnt[K];                             /* passing traces with T >= ns, per slot */
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -645,18 +644,18 @@ k_migrate(const MigArgs a)
         if (tl < ns) {
             const float v2 = a.vel2[vbase + a.t1[tl]];
             sV2c[tl] = v2;
-            r[k] = 1.0f / v2;
-            rs[k] = SCALED ? fast_rs(v2, a.dt) : 0.0f;
-            tq[k] = SCALED ? fast_tq(tl) : a.t2[tl];
+            if (RREG) r[k] = 1.0f / v2;
+            rs[k] = fast_rs(v2, a.dt);
+            tq[k] = fast_tq(tl);
             acc[k] = col_ok ? a.km[pix + tl] : 0.0f;
             if (KAHAN && col_ok) cmp[k] = a.comp[pix + tl];
         } else {
             sV2c[tl] = 1.0f;
             /* padding lane: q = 0.5, never flagged by the ambiguity test; it
              * reads F[0] into a value that is not stored */
-            r[k] = 0.0f;
+            if (RREG) r[k] = 0.0f;
             rs[k] = 0.0f;
-            tq[k] = SCALED ? 0.0625f : 0.0625f * a.dt * a.dt;
+            tq[k] = 0.0625f;
             acc[k] = 0.0f;
         }
     }
@@ -721,47 +720,63 @@ k_migrate(const MigArgs a)
                 }
                 uint64_t pm = __ballot(gp);
                 npass += __popcll(pm);
+                /* Hot loop, Kirchhoff.c:1374-1399: the tap F[T] of slot k for
+                 * one trace.  Fast path (fast_q) with the reference arithmetic
+                 * (exact_T) wherever an integer is within the fast path's
+                 * error bound of q, so T is always the reference's. */
+                auto tap = [&](int k, float dIS2, float dIR2, const float *Fr) __attribute__((always_inline)) {
+                    const float q = fast_qs(dIS2, dIR2, rs[k], tq[k]);
+                    /* clamped at ns: a sample at or past the end of the
+                     * trace lands on an integer and takes the rare path
+                     * with the ambiguous ones, where the samples that do
+                     * not count towards fold are counted, so the common
+                     * path has no compare for fold */
+                    constexpr float EPS = KTM_FAST_EPS_S;
+                    const float qc = fminf(q, nsf);
+                    int T = fast_T(qc);
+                    if (fast_ambiguous(qc, EPS)) {
+                        if (q - nsf >= __builtin_fmaf(q, EPS, EPS))
+                            T = ns;   /* Q >= ns as well: past the trace */
+                        else {
+                            const int tl = seg * K * KTM_WAVE + k * KTM_WAVE + lane;
+                            const float v2 = sV2c[tl];
+                            T = exact_T(dIS2, dIR2, a.t2[tl], v2, RREG ? r[RREG ? k : 0] : 1.0f / v2,
+                                        a.dt, a.rdt, ns);
+                        }
+                        cnt[k] += T >= ns ? 1 : 0;
+                    }
+                    return Fr[T];
+                };
+                /* Kahan variants: the taps of the batch's passing traces are
+                 * summed onto y = -c in plain FP32 (at most B terms, error
+                 * <= (B-1)u times their magnitudes) and added to the image
+                 * with one compensated step per batch, which keeps the
+                 * compensated error bound of order u * sum |f| at a fraction
+                 * of the per-tap cost.  Plain variants add every tap into
+                 * the image in trace order, as the reference does. */
+                float y[KAHAN ? K : 1];
+#pragma unroll
+                for (int k = 0; k < (KAHAN ? K : 1); ++k) y[k] = -cmp[k];
                 while (pm) {
                     const int j = __builtin_ctzll(pm);   /* traces in order */
                     pm &= pm - 1ull;
                     const float dIS2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g_is), j));
                     const float dIR2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g_ir), j));
                     const float *Fr = sF + j * ldl;
-                    /* Hot loop, Kirchhoff.c:1374-1399.  Fast path (fast_q) with
-                     * the reference arithmetic (exact_T) wherever an integer is
-                     * within the fast path's error bound of q, so T is always
-                     * the reference's. */
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
-                        const float q = SCALED ? fast_qs(dIS2, dIR2, rs[k], tq[k])
-                                               : fast_q(dIS2, dIR2, r[k], tq[k], a.rdt);
-                        /* clamped at ns: a sample at or past the end of the
-                         * trace lands on an integer and takes the rare path
-                         * with the ambiguous ones, where the samples that
-                         * do not count towards fold are counted, so the
-                         * common path has no compare for fold */
-                        constexpr float EPS = SCALED ? KTM_FAST_EPS_S : KTM_FAST_EPS;
-                        const float qc = fminf(q, nsf);
-                        int T = fast_T(qc);
-                        if (fast_ambiguous(qc, EPS)) {
-                            if (q - nsf >= __builtin_fmaf(q, EPS, EPS))
-                                T = ns;   /* Q >= ns as well: past the trace */
-                            else {
-                                const int tl = seg * K * KTM_WAVE + k * KTM_WAVE + lane;
-                                T = exact_T(dIS2, dIR2, SCALED ? a.t2[tl] : tq[k], sV2c[tl], r[k],
-                                            a.dt, a.rdt, ns);
-                            }
-                            cnt[k] += T >= ns ? 1 : 0;
-                        }
-                        const float f = Fr[T];
-                        if (KAHAN) {
-                            const float y = f - cmp[k];
-                            const float s = acc[k] + y;
-                            cmp[k] = (s - acc[k]) - y;
-                            acc[k] = s;
-                        } else {
-                            acc[k] = acc[k] + f;
-                        }
+                        const float f = tap(k, dIS2, dIR2, Fr);
+                        if (KAHAN) y[KAHAN ? k : 0] = y[KAHAN ? k : 0] + f;
+                        else acc[k] = acc[k] + f;
+                    }
+                }
+                if (KAHAN) {
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        const float yk = y[KAHAN ? k : 0];
+                        const float s = acc[k] + yk;
+                        cmp[k] = (s - acc[k]) - yk;
+                        acc[k] = s;
                     }
                 }
             }
@@ -1684,22 +1699,12 @@ extern "C" int ktm_image_reset(const ktm_plan *p, float *km, float *fold, float 
 /* kernel choice: K sample slots per lane and W waves per column cover ns */
 typedef void (*mig_fn)(const MigArgs);
 
-/* traces staged per batch: 8 with plain sums; 6 with the Kahan image, whose
- * extra registers leave less room (two workgroups per CU either way) */
-template <int K, int W>
+/* B traces staged per batch (8: two workgroups per CU fit the LDS) */
+template <int K, int W, int B_ = 8>
 static mig_fn pick_accum(int accum, int *B)
 {
-    *B = accum == KTM_ACCUM_PLAIN ? 8 : 6;
-    return accum == KTM_ACCUM_KAHAN ? k_migrate<K, W, 6, true> : k_migrate<K, W, 8, false>;
-}
-
-/* one column per workgroup (8 waves of K*64 samples) for long traces, with
- * fewer traces staged per batch so the rows fit the LDS */
-template <int K, int BP, int BK>
-static mig_fn pick_long(int accum, int *B)
-{
-    *B = accum == KTM_ACCUM_PLAIN ? BP : BK;
-    return accum == KTM_ACCUM_KAHAN ? k_migrate<K, 8, BK, true> : k_migrate<K, 8, BP, false>;
+    *B = B_;
+    return accum == KTM_ACCUM_KAHAN ? k_migrate<K, W, B_, true> : k_migrate<K, W, B_, false>;
 }
 
 /* W waves share one image column (each owns K*64 consecutive samples);
@@ -1708,14 +1713,17 @@ static mig_fn pick_migrate(int ns, int accum, int *W, int *K, int *B)
 {
     if (ns <= 512) { *W = 1; *K = 8; return pick_accum<8, 1>(accum, B); }
     if (ns <= 1024) { *W = 2; *K = 8; return pick_accum<8, 2>(accum, B); }
-    /* (the Kahan K = 12 variant spills some registers, yet runs at 0.87x the
-     * plain one; four waves of K = 6 per column measured 0.72x, of K = 8 0.63x) */
+    /* (the Kahan K = 12 variant runs at 0.96x the plain one; four waves of
+     * K = 6 per column measured 0.72x, of K = 8 0.63x, before the Kahan sums
+     * were blocked per batch) */
     if (ns <= 1536) { *W = 2; *K = 12; return pick_accum<12, 2>(accum, B); }
     if (ns <= 2048) { *W = 4; *K = 8; return pick_accum<8, 4>(accum, B); }
     if (ns <= 3072) { *W = 4; *K = 12; return pick_accum<12, 4>(accum, B); }
-    if (ns <= 4096) { *W = 8; *K = 8; return pick_long<8, 8, 6>(accum, B); }
+    /* one column per workgroup (8 waves of K*64 samples) for long traces,
+     * with fewer traces staged per batch so the rows fit the LDS */
+    if (ns <= 4096) { *W = 8; *K = 8; return pick_accum<8, 8>(accum, B); }
     *W = 8; *K = 12;
-    return pick_long<12, 5, 4>(accum, B);
+    return pick_accum<12, 8, 5>(accum, B);
 }
 
 /* FP32 filtered traces of a BF16/FP16 block, in a workspace the plan owns and
