@@ -34,10 +34,10 @@ BYTES_FP32 = 12.0   # three FP32 trace taps per contribution
 BYTES_HALF = 6.0    # three 16-bit taps
 METRIC = "G trace×image-pt contributions/s, %HBM roofline; BF16-trace vs FP32 at err"
 # HBM bytes per launch of k_migrate on this workload, from rocprofv3 --pmc
-# FETCH_SIZE (454,294,059 KiB, scaled x2.0) + WRITE_SIZE (1,881,157 KiB, x1.0),
+# FETCH_SIZE (450,847,687 KiB, scaled x2.0) + WRITE_SIZE (1,881,311 KiB, x1.0),
 # the scale factors being what the same counters read for a 1 GiB device copy
 # (profiles/k_migrate_area3d_1e6.md)
-TRAFFIC = 932_320_537_600
+TRAFFIC = 925_262_524_992
 
 
 def log(*a):
