@@ -54,13 +54,15 @@ def parse():
     ap.add_argument("--ncols", type=int, default=400)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline time")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the ingest, velocity-gradient and EOM legs")
+    ap.add_argument("--no-extra", action="store_true", help="skip the ingest, velocity-gradient, adjoint/LSM and EOM legs")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3")
     ap.add_argument("--leg-steps", type=int, default=3, help="timed blocks per precision leg")
     ap.add_argument("--err-traces", type=int, default=20000,
                     help="traces of the FP64-oracle error check on the 50x50x1500 sub-volume")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy 3x (HBM counter calibration) and exit")
+    ap.add_argument("--leg", choices=("vgrad", "adjoint", "eom"),
+                    help="only run this extra leg, print its JSON and exit (for profiling)")
     return ap.parse_args()
 
 
@@ -284,6 +286,66 @@ def vgrad_leg(dev, reps=3, fd_nodes=10):
             "fd_max_rel_err": float(rel.max()), "fd_median_rel_err": float(np.median(rel))}
 
 
+def adjoint_leg(dev, reps=3, niter=20):
+    """The adjoint pair and least-squares migration on the 200 x 240 line
+    (48,000 traces x 1501 samples, 1600 columns): one application of L^T
+    (linear migration: closed-form filter, compensated sums) and of L
+    (demigration), the dot-product test on seeded Gaussian d and m, and
+    niter CGLS iterations (time-gain preconditioned) from data modelled by L
+    on 50 point scatterers."""
+    import numpy as np
+    import torch
+
+    from random_amd import surveys
+    from random_amd.ktm import Plan
+    from random_amd.lsm import Operator, cgls, dot64, dot_test, time_gain
+    sv = surveys.line2d_lsm()
+    plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
+    co = torch.from_numpy(sv.coords).to(dev)
+    op = Operator(plan, co)
+    g = torch.Generator(device=dev).manual_seed(1)
+    d = torch.randn((sv.ntr, sv.ns), generator=g, device=dev)
+    m = torch.randn(plan.image_shape, generator=g, device=dev)
+    n = plan.migrate(d, co)[2]
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    ms_a, ms_f = [], []
+    for _ in range(reps + 1):
+        torch.cuda.synchronize()
+        e0.record()
+        op.adjoint(d)
+        e1.record()
+        op.forward(m)
+        e2.record()
+        torch.cuda.synchronize()
+        ms_a.append(e0.elapsed_time(e1))
+        ms_f.append(e1.elapsed_time(e2))
+    ms_a, ms_f = float(np.median(ms_a[1:])), float(np.median(ms_f[1:]))
+    _, _, rel = dot_test(op, d, m)
+    m_true = torch.from_numpy(surveys.point_reflectivity(sv, plan.image_shape)).to(dev)
+    d_obs = op.forward(m_true)
+    w = time_gain(op, plan.image_shape, nprobe=4, seed=5)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    m_est, hist = cgls(op, d_obs, niter=niter, precond=w)
+    torch.cuda.synchronize()
+    ms_cg = (time.perf_counter() - t0) * 1e3
+    mig = op.adjoint(d_obs)
+    corr = dot64(m_est, m_true) / (dot64(m_est, m_est) * dot64(m_true, m_true)) ** 0.5
+    corr0 = dot64(mig, m_true) / (dot64(mig, mig) * dot64(m_true, m_true)) ** 0.5
+    log(f"adjoint: L^T {ms_a:.2f} ms, L {ms_f:.2f} ms, {n:.3e} contributions, dot rel {rel:.2e}, "
+        f"CGLS {niter} it {ms_cg:.0f} ms, residual {hist[-1][0] / hist[0][0]:.3f}")
+    return {"workload": "2-D line 200 shots x 240 receivers (48,000 traces), 1501 samples, "
+                        "1600 columns; 50 point scatterers",
+            "contributions_per_application": int(n),
+            "ms_migration_adjoint": round(ms_a, 2), "ms_demigration": round(ms_f, 2),
+            "G_contributions_per_s_migration": round(n / (ms_a * 1e-3) / 1e9, 1),
+            "G_contributions_per_s_demigration": round(n / (ms_f * 1e-3) / 1e9, 1),
+            "dot_test_rel": float(rel), "cgls_iterations": niter,
+            "cgls_ms": round(ms_cg, 1),
+            "cgls_relative_residual": round(hist[-1][0] / hist[0][0], 4),
+            "correlation_with_truth": {"migration": round(corr0, 4), "lsm": round(corr, 4)}}
+
+
 def eom_leg(dev, reps=3, cpu_traces=48):
     """Equivalent-offset migration (EOM.c csp()) of the configuration-0 line:
     4800 traces x 1001 samples into 800 CSPs x 160 offset bins, plus the post
@@ -333,6 +395,11 @@ def main():
     args = parse()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.leg:
+        import torch
+        fn = {"vgrad": vgrad_leg, "adjoint": adjoint_leg, "eom": eom_leg}[args.leg]
+        print(json.dumps(fn(torch.device("cuda:0"))), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "runs on one GPU only; across GPUs each would migrate its "
@@ -460,6 +527,7 @@ def main():
     if not args.profile and not args.no_extra:
         result["ingest"] = ingest_leg(sv, plan, traces, args.block, dev)
         result["velocity_gradient"] = vgrad_leg(dev)
+        result["adjoint_lsm"] = adjoint_leg(dev)
         result["eom"] = eom_leg(dev)
     if not args.no_cpu and not args.profile:
         nthreads = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))

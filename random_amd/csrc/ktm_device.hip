@@ -28,6 +28,7 @@
  * divide/sqrt; packed-FP32 ops are disabled.
  */
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <stdint.h>
 #include <stdlib.h>
@@ -57,7 +58,14 @@ struct ktm_plan {
     double *d_hconv2;       /* [2 ns] post-stack kernel, h2[m] = h[m mod ns] */
     float *d_stage;         /* FP32 filtered traces of a BF16/FP16 block */
     size_t stage_cap;       /* elements of d_stage */
-    float *d_pix_v2, *d_pix_r;  /* per-pixel v^2 and RN(1/v^2), built for demigration */
+    float *d_pix_v2, *d_pix_rs; /* per-pixel v^2 and RN(1/(v^2 dt^2)), built for demigration */
+    /* demigration trace order: sort keys and indices (in, out) and the
+     * radix sort's temporary storage, grown on demand */
+    uint64_t *d_dm_keys;
+    int32_t *d_dm_idx;
+    void *d_dm_tmp;
+    size_t dm_cap, dm_tmp_bytes;
+    uint32_t *d_dm_amax;    /* bits of max |km| of the last demigration */
 };
 
 #define HIP_TRY(expr)                                  \
@@ -1096,12 +1104,13 @@ extern "C" int ktm_selftest_traveltime(const float *in4, int64_t n, float dt, in
 /* ------------------------------------------------------------------------ */
 /* adjoint pair: demigration (L) and the post-stack transpose                */
 /* ------------------------------------------------------------------------ */
-/* Per-pixel velocity tables for trace-parallel kernels: v2 and RN(1/v2) of
- * every (column, sample), the same values k_migrate keeps per lane. */
+/* Per-pixel velocity tables for trace-parallel kernels: v2 and rs =
+ * RN(1/(v2 dt^2)) of every (column, sample), the values k_migrate derives
+ * per lane. */
 __global__ void k_pixel_velocity(const float *__restrict__ vel2, const int32_t *__restrict__ velx,
                                  const int32_t *__restrict__ vely, const int32_t *__restrict__ t1,
-                                 int nxp, int nyp, int ns, int nvel_xl, int nsm,
-                                 float *__restrict__ pv2, float *__restrict__ pr)
+                                 int nxp, int nyp, int ns, int nvel_xl, int nsm, float dt,
+                                 float *__restrict__ pv2, float *__restrict__ prs)
 {
     const int64_t n = (int64_t)nxp * nyp * ns;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
@@ -1111,105 +1120,240 @@ __global__ void k_pixel_velocity(const float *__restrict__ vel2, const int32_t *
         const int X = (int)(c / nyp), Y = (int)(c % nyp);
         const float v2 = vel2[((int64_t)velx[X] * nvel_xl + vely[Y]) * nsm + t1[t]];
         pv2[e] = v2;
-        pr[e] = 1.0f / v2;
+        prs[e] = fast_rs(v2, dt);
     }
 }
+
+#define DM_TB 8                      /* traces per demigration workgroup, at most */
 
 struct DemigArgs {
     const float *km;                 /* image [nxp][nyp][nh][ns] */
     const ktm_trace_geom *geom;
+    const int32_t *order;            /* trace order (workgroup b takes order[b*tb ...]) or NULL */
     int64_t ntr;
     float *traces;                   /* out [ntr][ns] */
-    const float *pv2, *pr, *t2, *w, *colx, *coly, *slab_cx, *slab_cy, *slab_x2thr;
+    const float *pv2, *prs, *t2, *w, *colx, *coly, *slab_cx, *slab_cy, *slab_x2thr;
+    const uint32_t *amax;            /* bits of max |km| (k_absmax) */
     float ap_x2thr, aperture, dt, rdt;
-    int ns, nh, nxp, nyp, nx, ny, jobs_y;
+    int ns, nh, nxp, nyp, nx, ny, jobs_y, tb;
+    int lg_terms;                    /* 2^lg_terms >= nxp * nyp * ns, terms per g[T] at most */
 };
 
+/* *out = bits of max |x| over n floats (out zeroed by k_zero_u32 first;
+ * non-negative floats order like their bit patterns) */
+__global__ void k_zero_u32(uint32_t *out) { *out = 0u; }
+__global__ void k_absmax(const float *__restrict__ x, int64_t n, uint32_t *out)
+{
+    uint32_t m = 0u;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x)
+        m = max(m, __float_as_uint(x[e]) & 0x7fffffffu);
+    for (int d = KTM_WAVE / 2; d >= 1; d >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, d));
+    if ((threadIdx.x & (KTM_WAVE - 1)) == 0) atomicMax(out, m);
+}
+
+/* Column index range whose centres can be within the aperture of a CMP at
+ * coordinate c on an axis of n columns from col[0] to col[n-1]. */
+__device__ __forceinline__ void aperture_span(const float *col, int n, float c, float aperture,
+                                              int *lo, int *hi)
+{
+    *lo = 0;
+    *hi = n - 1;
+    if (n > 1) {
+        const float step = (col[n - 1] - col[0]) / (float)(n - 1);
+        if (step != 0.0f) {
+            const float xc = (c - col[0]) / step, half = aperture / fabsf(step) + 2.0f;
+            *lo = max(0, (int)floorf(xc - half));
+            *hi = min(n - 1, (int)ceilf(xc + half));
+        }
+    }
+}
+
+/* Sort keys of the demigration trace order: offset class (out-of-class
+ * traces last), then the CMP's image cell, row by row, so the traces one
+ * workgroup takes lie close together and share their columns. */
+__global__ void k_demig_keys(const ktm_trace_geom *__restrict__ geom, int64_t ntr, int nh,
+                             const float *__restrict__ colx, int nxp, const float *__restrict__ coly,
+                             int nyp, uint64_t *__restrict__ keys, int32_t *__restrict__ idx)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntr) return;
+    const ktm_trace_geom g = geom[i];
+    const uint64_t cls = g.off >= 0 && g.off < nh ? (uint64_t)g.off : (uint64_t)nh;
+    uint64_t cell[2] = {0, 0};
+    const float *col[2] = {colx, coly};
+    const int n[2] = {nxp, nyp};
+    const float c[2] = {g.cmpx, g.cmpy};
+    for (int d = 0; d < 2; ++d) {
+        if (n[d] < 2) continue;
+        const float step = (col[d][n[d] - 1] - col[d][0]) / (float)(n[d] - 1);
+        if (step == 0.0f) continue;
+        /* fminf/fmaxf return the number for a NaN operand */
+        const float f = fminf(fmaxf(floorf((c[d] - col[d][0]) / step), -1.0f), (float)n[d]);
+        cell[d] = (uint64_t)((int)f + 1);
+    }
+    keys[i] = (cls << 42) | (cell[1] << 21) | cell[0];
+    idx[i] = (int32_t)i;
+}
+
 /* L, the exact transpose of migration with the CLOSED filter (km = L^T d).
- * One workgroup per trace: the waves walk the image columns of the box around
- * the trace's aperture disc, gate them exactly like k_migrate, and for every
- * sample t of a passing column with T(t) < ns add km[column][off][t] into
- * g[T] (LDS atomics).  The trace is then the transpose of the 3-tap filter
- * and of the sample weights applied to g. */
+ * A workgroup takes up to DM_TB traces, neighbours in the sort order of
+ * k_demig_keys: the waves walk the image columns of the union of their
+ * aperture boxes, gate every trace for the column exactly like k_migrate
+ * (lane j = trace j), and for each passing trace and every sample t with
+ * T(t) < ns add km[column][off][t] into that trace's g[T] (LDS atomics).
+ * The column's image samples and rs are loaded once for all its traces.
+ * g is 64-bit fixed point, km scaled by 2^E with max |km| < 2^(59 - E -
+ * lg_terms), so no sum of at most 2^lg_terms terms passes 2^59 and the
+ * filter's combinations of six stay below 2^62: integer LDS
+ * atomics run at the rate of plain stores where float ones serialise over
+ * the lanes (measured 22 vs 193 ms on the 48,000-trace line), and the
+ * integer sums make L deterministic.  The quantum is 2^-E, at most
+ * max |km| * 2^(lg_terms - 59).
+ * Each trace is then the transpose of the 3-tap filter and of the sample
+ * weights applied to its g.
+ * Lane l owns samples l*K + ((k + l) mod K) of its wave's 64K-sample
+ * segment, not k*64 + l: far from the CMP, T(t) grows slowly at early
+ * times, and 64 consecutive samples land on a few T, whose atomics then
+ * serialise; samples ~13 apart land on distinct T in distinct banks. */
 template <int K, int W>
-__global__ void __launch_bounds__(8 * KTM_WAVE) k_demigrate(const DemigArgs a)
+__global__ void __launch_bounds__(8 * KTM_WAVE) __attribute__((amdgpu_waves_per_eu(4, 4)))
+k_demigrate(const DemigArgs a)
 {
     constexpr int NCOL = 8 / W;
-    extern __shared__ float sg[];     /* [ns + 4] */
-    const int64_t i = blockIdx.x;
+    extern __shared__ unsigned long long sg[];     /* [tb][ns + 4] fixed point */
+    __shared__ int32_t sIdx[DM_TB];
     const int tid = threadIdx.x, lane = tid & (KTM_WAVE - 1), wv = tid / KTM_WAVE;
     const int cw = wv / W, seg = wv % W;
-    const int ns = a.ns;
-    for (int t = tid; t < ns + 4; t += 8 * KTM_WAVE) sg[t] = 0.0f;
+    const int ns = a.ns, ldg = ns + 4;
+    const int64_t b0 = (int64_t)blockIdx.x * a.tb;
+    const int nb = (int)min((int64_t)a.tb, a.ntr - b0);
+    for (int e = tid; e < a.tb * ldg; e += 8 * KTM_WAVE) sg[e] = 0ull;
+    const float amax = __uint_as_float(*a.amax);
+    int emax = 0;                          /* max |km| < 2^emax */
+    if (amax > 0.0f) (void)frexpf(amax, &emax);
+    const int E = 59 - emax - a.lg_terms;
+    /* lane j < nb of every wave holds trace j of the workgroup */
+    ktm_trace_geom g;
+    memset(&g, 0, sizeof g);
+    bool valid = false;
+    int gi = 0;
+    int xlo = a.nxp, xhi = -1, ylo = a.nyp, yhi = -1;
+    if (lane < nb) {
+        gi = a.order ? a.order[b0 + lane] : (int)(b0 + lane);
+        g = a.geom[gi];
+        valid = g.off >= 0 && g.off < a.nh;
+        if (valid) {
+            aperture_span(a.colx, a.nxp, g.cmpx, a.aperture, &xlo, &xhi);
+            aperture_span(a.coly, a.nyp, g.cmpy, a.aperture, &ylo, &yhi);
+        }
+    }
+    if (tid < nb) sIdx[tid] = gi;
+    for (int m = KTM_WAVE / 2; m >= 1; m >>= 1) {
+        xlo = min(xlo, __shfl_xor(xlo, m));
+        xhi = max(xhi, __shfl_xor(xhi, m));
+        ylo = min(ylo, __shfl_xor(ylo, m));
+        yhi = max(yhi, __shfl_xor(yhi, m));
+    }
     __syncthreads();
-    const ktm_trace_geom g = a.geom[i];
     const float nsh = (float)ns + 0.5f;   /* clamp of q */
-    if (g.off >= 0 && g.off < a.nh) {
-        /* columns whose centre can be within the aperture of the CMP */
-        int xlo = 0, xhi = a.nxp - 1, ylo = 0, yhi = a.nyp - 1;
-        if (a.nxp > 1) {
-            const float step = (a.colx[a.nxp - 1] - a.colx[0]) / (float)(a.nxp - 1);
-            if (step != 0.0f) {
-                const float xc = (g.cmpx - a.colx[0]) / step, half = a.aperture / fabsf(step) + 2.0f;
-                xlo = max(0, (int)floorf(xc - half));
-                xhi = min(a.nxp - 1, (int)ceilf(xc + half));
-            }
-        }
-        if (a.nyp > 1) {
-            const float step = (a.coly[a.nyp - 1] - a.coly[0]) / (float)(a.nyp - 1);
-            if (step != 0.0f) {
-                const float yc = (g.cmpy - a.coly[0]) / step, half = a.aperture / fabsf(step) + 2.0f;
-                ylo = max(0, (int)floorf(yc - half));
-                yhi = min(a.nyp - 1, (int)ceilf(yc + half));
-            }
-        }
-        const int nbx = xhi - xlo + 1, nby = yhi - ylo + 1;
-        const int64_t ncols = nbx > 0 && nby > 0 ? (int64_t)nbx * nby : 0;
-        for (int64_t cidx = cw; cidx < ncols; cidx += NCOL) {
-            const int X = xlo + (int)(cidx / nby), Y = ylo + (int)(cidx % nby);
-            const float IX = a.colx[X], IY = a.coly[Y];
-            const int job = (X / a.nx) * a.jobs_y + (Y / a.ny);
+    int rot = lane % K;
+    const int tbase = seg * K * KTM_WAVE + lane * K;
+    auto slot_t = [&](int k) __attribute__((always_inline)) {   /* sample of slot k */
+        return tbase + (k + rot < K ? k + rot : k + rot - K);
+    };
+    /* past the trace, tq puts q at ns + 1: clamped to ns + 0.5, T = ns, no add */
+    const float tq_past = 0.25f * ((float)ns + 1.0f) * ((float)ns + 1.0f);
+    float tq[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int t = slot_t(k);
+        tq[k] = t < ns ? fast_tq(t) : tq_past;
+    }
+    const int nbx = xhi - xlo + 1, nby = yhi - ylo + 1;
+    const int64_t ncols = nbx > 0 && nby > 0 ? (int64_t)nbx * nby : 0;
+    for (int64_t cidx = cw; cidx < ncols; cidx += NCOL) {
+        const int X = xlo + (int)(cidx / nby), Y = ylo + (int)(cidx % nby);
+        const float IX = a.colx[X], IY = a.coly[Y];
+        const int job = (X / a.nx) * a.jobs_y + (Y / a.ny);
+        float g_is = 0.0f, g_ir = 0.0f;
+        bool gp = false;
+        if (valid) {
             const float ex = a.slab_cx[job] - g.cmpx, ey = a.slab_cy[job] - g.cmpy;
             const float cx = IX - g.cmpx, cy = IY - g.cmpy;
-            const bool pass = ex * ex + ey * ey < a.slab_x2thr[job] && cx * cx + cy * cy < a.ap_x2thr;
-            if (!__builtin_amdgcn_readfirstlane((int)pass)) continue;
+            gp = ex * ex + ey * ey < a.slab_x2thr[job] && cx * cx + cy * cy < a.ap_x2thr;
             const float ax = IX - g.sxt, ay = IY - g.syt, bx = IX - g.gxt, by = IY - g.gyt;
-            const float dIS2 = ax * ax + ay * ay, dIR2 = bx * bx + by * by;
-            const int64_t pixc = ((int64_t)X * a.nyp + Y) * ns;
-            const float *mk = a.km + (((int64_t)X * a.nyp + Y) * a.nh + g.off) * ns;
+            g_is = ax * ax + ay * ay;
+            g_ir = bx * bx + by * by;
+        }
+        uint64_t pm = __ballot(gp);
+        if (!pm) continue;
+        /* the slot samples and their addresses are recomputed where used
+         * rather than held in registers across the loops (an empty asm
+         * hides that rot is loop-invariant) */
+        asm volatile("" : "+v"(rot));
+        const int64_t pixc = ((int64_t)X * a.nyp + Y) * ns;
+        float rs[K];
+        long long mq[K];                   /* km samples, fixed point */
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int t = slot_t(k);
+            rs[k] = t < ns ? a.prs[pixc + t] : 0.0f;
+            mq[k] = 0;
+        }
+        int cur = -1;                      /* offset class whose samples mq holds */
+        while (pm) {
+            const int j = __builtin_ctzll(pm);
+            pm &= pm - 1ull;
+            const float dIS2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g_is), j));
+            const float dIR2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g_ir), j));
+            const int offj = __builtin_amdgcn_readlane(g.off, j);
+            asm volatile("" : "+v"(rot));
+            if (offj != cur) {
+                cur = offj;
+                const float *mk = a.km + (((int64_t)X * a.nyp + Y) * a.nh + offj) * ns;
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const int t = slot_t(k);
+                    mq[k] = t < ns ? (long long)ldexpf(mk[t], E) : 0;
+                }
+            }
+            unsigned long long *gl = sg + j * ldg;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                const int t = seg * K * KTM_WAVE + k * KTM_WAVE + lane;
-                if (t < ns) {
-                    const float r = a.pr[pixc + t], t2 = a.t2[t];
-                    const float qc = fast_clamp(fast_q(dIS2, dIR2, r, t2, a.rdt), nsh);
-                    int T = fast_T(qc);
-                    if (fast_ambiguous(qc))
-                        T = exact_T(dIS2, dIR2, t2, a.pv2[pixc + t], r, a.dt, a.rdt, ns);
-                    if (T < ns) atomicAdd(&sg[T], mk[t]);
+                const float qc = fast_clamp(fast_qs(dIS2, dIR2, rs[k], tq[k]), nsh);
+                int T = fast_T(qc);
+                if (fast_ambiguous(qc, KTM_FAST_EPS_S)) {   /* never past the trace */
+                    const int t = slot_t(k);
+                    const float v2 = a.pv2[pixc + t];
+                    T = exact_T(dIS2, dIR2, a.t2[t], v2, 1.0f / v2, a.dt, a.rdt, ns);
                 }
+                if (T < ns) atomicAdd(&gl[T], (unsigned long long)mq[k]);
             }
         }
     }
     __syncthreads();
-    /* transpose of F = closed_tap(x), x[t] = tr[min(t, ns-1)] * w[t] */
-    float *out = a.traces + i * (int64_t)ns;
-    for (int t = tid; t < ns; t += 8 * KTM_WAVE) {
-        float acc = 0.0f;
+    /* transpose of F = closed_tap(x), x[t] = tr[min(t, ns-1)] * w[t], in
+     * double from the fixed-point sums */
+    const double q = ldexp(1.0, -E);
+    for (int e = tid; e < nb * ns; e += 8 * KTM_WAVE) {
+        const int j = e / ns, t = e - j * ns;
+        const long long *gl = reinterpret_cast<const long long *>(sg + j * ldg);
+        double acc = 0.0;
         const int nt_ = t == ns - 1 ? 3 : 1;   /* tr[ns-1] also feeds x[ns], x[ns+1] */
         for (int u = 0; u < nt_; ++u) {
             const int tt = t + u;
-            float xb = 0.0f;
-            if (tt >= 2 && tt < ns) xb += sg[tt];
-            if (tt - 1 >= 2 && tt - 1 < ns) xb += 2.0f * sg[tt - 1];
-            if (tt - 2 >= 2 && tt - 2 < ns) xb += sg[tt - 2];
-            if (tt == 0) xb += 4.0f * sg[0] + 2.0f * sg[1];
-            if (tt == 1) xb += 4.0f * sg[0] + 2.0f * sg[1];
-            if (tt == 2) xb += 2.0f * sg[0] + 2.0f * sg[1];
-            if (tt == 3) xb += sg[1];
-            acc += a.w[tt] * xb;
+            long long xb = 0;                  /* exact: |g| < 2^59, weights summing to <= 6 */
+            if (tt >= 2 && tt < ns) xb += gl[tt];
+            if (tt - 1 >= 2 && tt - 1 < ns) xb += 2 * gl[tt - 1];
+            if (tt - 2 >= 2 && tt - 2 < ns) xb += gl[tt - 2];
+            if (tt == 0) xb += 4 * gl[0] + 2 * gl[1];
+            if (tt == 1) xb += 4 * gl[0] + 2 * gl[1];
+            if (tt == 2) xb += 2 * gl[0] + 2 * gl[1];
+            if (tt == 3) xb += gl[1];
+            acc += (double)a.w[tt] * ((double)xb * q);
         }
-        out[t] = acc;
+        a.traces[(int64_t)sIdx[j] * ns + t] = (float)acc;
     }
 }
 
@@ -1293,7 +1437,8 @@ extern "C" void ktm_plan_destroy(ktm_plan *p)
     if (!p) return;
     void *bufs[] = {p->d_t2, p->d_t1, p->d_colx, p->d_coly, p->d_velx, p->d_vely,
                     p->d_slab_cx, p->d_slab_cy, p->d_slab_x2thr, p->d_vel2, p->d_w, p->d_hconv2,
-                    p->d_stage, p->d_pix_v2, p->d_pix_r};
+                    p->d_stage, p->d_pix_v2, p->d_pix_rs, p->d_dm_keys, p->d_dm_idx,
+                    p->d_dm_tmp, p->d_dm_amax};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     ktm_tables_free(&p->tb);
@@ -1927,15 +2072,57 @@ static int pixel_tables(const ktm_plan *cp, hipStream_t hs)
     const ktm_setup *s = &p->tb.s;
     const size_t n = (size_t)s->nxp * s->nyp * s->ns;
     if (hipMalloc((void **)&p->d_pix_v2, n * sizeof(float)) != hipSuccess) return KTM_ERR_NOMEM;
-    if (hipMalloc((void **)&p->d_pix_r, n * sizeof(float)) != hipSuccess) {
+    if (hipMalloc((void **)&p->d_pix_rs, n * sizeof(float)) != hipSuccess) {
         (void)hipFree(p->d_pix_v2);
         p->d_pix_v2 = nullptr;
         return KTM_ERR_NOMEM;
     }
     hipLaunchKernelGGL(k_pixel_velocity, dim3(4096), dim3(256), 0, hs, p->d_vel2, p->d_velx,
                        p->d_vely, p->d_t1, s->nxp, s->nyp, s->ns, p->cfg.nvel_xlines, s->nsm,
-                       p->d_pix_v2, p->d_pix_r);
+                       s->dt, p->d_pix_v2, p->d_pix_rs);
     HIP_TRY(hipGetLastError());
+    return KTM_OK;
+}
+
+/* the demigration trace order of n traces (sorted by k_demig_keys) into
+ * plan-owned buffers; *order points at the sorted indices */
+static int demig_order(const ktm_plan *cp, const ktm_trace_geom *geom, int64_t n, hipStream_t hs,
+                       const int32_t **order)
+{
+    ktm_plan *p = const_cast<ktm_plan *>(cp);
+    const ktm_setup *s = &p->tb.s;
+    if (n > 0x7fffffffLL || s->nh >= (1 << 21)) return KTM_ERR_UNSUPPORTED;
+    if ((size_t)n > p->dm_cap) {
+        if (p->d_dm_keys) HIP_TRY(hipFree(p->d_dm_keys));
+        if (p->d_dm_idx) HIP_TRY(hipFree(p->d_dm_idx));
+        p->d_dm_keys = nullptr;
+        p->d_dm_idx = nullptr;
+        p->dm_cap = 0;
+        if (hipMalloc((void **)&p->d_dm_keys, 2 * (size_t)n * sizeof(uint64_t)) != hipSuccess)
+            return KTM_ERR_NOMEM;
+        if (hipMalloc((void **)&p->d_dm_idx, 2 * (size_t)n * sizeof(int32_t)) != hipSuccess)
+            return KTM_ERR_NOMEM;
+        p->dm_cap = (size_t)n;
+    }
+    uint64_t *k_in = p->d_dm_keys, *k_out = p->d_dm_keys + n;
+    int32_t *i_in = p->d_dm_idx, *i_out = p->d_dm_idx + n;
+    size_t need = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, need, k_in, k_out, i_in, i_out, (int)n, 0,
+                                               64, hs));
+    if (need > p->dm_tmp_bytes) {
+        if (p->d_dm_tmp) HIP_TRY(hipFree(p->d_dm_tmp));
+        p->d_dm_tmp = nullptr;
+        p->dm_tmp_bytes = 0;
+        if (hipMalloc(&p->d_dm_tmp, need) != hipSuccess) return KTM_ERR_NOMEM;
+        p->dm_tmp_bytes = need;
+    }
+    hipLaunchKernelGGL(k_demig_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, hs, geom, n,
+                       s->nh, p->d_colx, s->nxp, p->d_coly, s->nyp, k_in, i_in);
+    HIP_TRY(hipGetLastError());
+    size_t have = p->dm_tmp_bytes;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(p->d_dm_tmp, have, k_in, k_out, i_in, i_out, (int)n,
+                                               0, 64, hs));
+    *order = i_out;
     return KTM_OK;
 }
 
@@ -1949,16 +2136,36 @@ extern "C" int ktm_demigrate(const ktm_plan *p, const float *km, const ktm_trace
     int st = pixel_tables(p, hs);
     if (st != KTM_OK) return st;
     const ktm_setup *s = &p->tb.s;
+    /* cell indices of the sort key take 21 bits per axis */
+    if (s->nxp >= (1 << 21) - 2 || s->nyp >= (1 << 21) - 2) return KTM_ERR_UNSUPPORTED;
     DemigArgs a;
     memset(&a, 0, sizeof a);
     a.km = km; a.geom = geom; a.ntr = ntr; a.traces = traces;
-    a.pv2 = p->d_pix_v2; a.pr = p->d_pix_r; a.t2 = p->d_t2; a.w = p->d_w;
+    a.pv2 = p->d_pix_v2; a.prs = p->d_pix_rs; a.t2 = p->d_t2; a.w = p->d_w;
     a.colx = p->d_colx; a.coly = p->d_coly;
     a.slab_cx = p->d_slab_cx; a.slab_cy = p->d_slab_cy; a.slab_x2thr = p->d_slab_x2thr;
     a.ap_x2thr = p->tb.ap_x2thr; a.aperture = (float)p->cfg.aperture;
     a.dt = s->dt; a.rdt = 1.0f / s->dt;
     a.ns = s->ns; a.nh = s->nh; a.nxp = s->nxp; a.nyp = s->nyp; a.nx = s->nx; a.ny = s->ny;
     a.jobs_y = p->cfg.jobs_y;
+    /* fixed-point scale of g: max |km| and a bound on the terms per g[T] */
+    a.lg_terms = 0;
+    while ((1ull << a.lg_terms) < (unsigned long long)s->nxp * s->nyp * s->ns) ++a.lg_terms;
+    if (!p->d_dm_amax) {
+        ktm_plan *pm = const_cast<ktm_plan *>(p);
+        if (hipMalloc((void **)&pm->d_dm_amax, sizeof(uint32_t)) != hipSuccess) return KTM_ERR_NOMEM;
+    }
+    a.amax = p->d_dm_amax;
+    const int64_t nimg = (int64_t)s->nxp * s->nyp * s->nh * s->ns;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, hs, p->d_dm_amax);
+    hipLaunchKernelGGL(k_absmax, dim3(1024), dim3(256), 0, hs, km, nimg, p->d_dm_amax);
+    /* traces per workgroup: DM_TB, fewer so that two workgroups share a CU
+     * (g rows of 8 bytes per sample within 76 KB), or one beyond that */
+    const size_t row = sizeof(unsigned long long) * (size_t)(s->ns + 4);
+    a.tb = (int)((76 * 1024) / row);
+    if (a.tb < 2) a.tb = (int)((152 * 1024) / row);
+    if (a.tb > DM_TB) a.tb = DM_TB;
+    if (a.tb < 1) return KTM_ERR_UNSUPPORTED;
     typedef void (*fn_t)(const DemigArgs);
     fn_t fn;
     if (s->ns <= 512) fn = k_demigrate<8, 1>;
@@ -1968,14 +2175,17 @@ extern "C" int ktm_demigrate(const ktm_plan *p, const float *km, const ktm_trace
     else if (s->ns <= 3072) fn = k_demigrate<12, 4>;
     else if (s->ns <= 4096) fn = k_demigrate<8, 8>;
     else fn = k_demigrate<12, 8>;
-    const size_t lds = sizeof(float) * (size_t)(s->ns + 4);
+    const size_t lds = row * (size_t)a.tb;
+    HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     for (int64_t i0 = 0; i0 < ntr; i0 += 1 << 30) {
         const int64_t n = ntr - i0 < (1 << 30) ? ntr - i0 : (1 << 30);
         DemigArgs b = a;
         b.geom = geom + i0;
         b.traces = traces + i0 * s->ns;
         b.ntr = n;
-        hipLaunchKernelGGL(fn, dim3((unsigned)n), dim3(8 * KTM_WAVE), lds, hs, b);
+        if ((st = demig_order(p, b.geom, n, hs, &b.order)) != KTM_OK) return st;
+        const int64_t nblk = (n + a.tb - 1) / a.tb;
+        hipLaunchKernelGGL(fn, dim3((unsigned)nblk), dim3(8 * KTM_WAVE), lds, hs, b);
     }
     HIP_TRY(hipGetLastError());
     return KTM_OK;

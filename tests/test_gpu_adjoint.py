@@ -123,3 +123,52 @@ def test_lsm_configuration(cuda):
     assert all(b <= a * (1 + 1e-6) for a, b in zip(res_pc, res_pc[1:]))
     assert res_pc[-1] < 0.65 * res[-1]
     assert corr_pc > 2.0 * corr
+
+
+@pytest.mark.parametrize("nh,dh,nrecv,jobs_x", [(1, 30.0, 2, 1), (2, 15.0, 3, 2)])
+def test_demigration_against_oracle_transpose(cuda, nh, dh, nrecv, jobs_x):
+    """L m against the transpose of the oracle's migration matrix: column
+    (i, t) of L^T is the oracle's image of a unit impulse at sample t of
+    trace i (FP64 data path with the reference's FP32 traveltime index), so
+    (L m)[i, t] = <L^T e_it, m>, accumulated in FP64.  Two offset classes
+    (offsets 12.5, 25 and 37.5 m: the last outside them) and two job slabs
+    in the second case."""
+    import oracle
+    sv = surveys.line2d(nshots=2, nrecv_side=nrecv, shot_dx=20.0, ns=96, ncols=12, nscat=1,
+                        x0=45.0, nh=nh, dh=dh, jobs_x=jobs_x)
+    plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
+    rng = np.random.default_rng(17)
+    m = rng.standard_normal(plan.image_shape).astype(np.float32)
+    ref = np.zeros((sv.ntr, sv.ns))
+    for i in range(sv.ntr):
+        for t in range(sv.ns):
+            e = np.zeros((1, sv.ns), np.float32)
+            e[0, t] = 1.0
+            col = oracle.migrate(sv.cfg, sv.ns, sv.dt, e, sv.coords[i:i + 1], sv.vel,
+                                 mode=oracle.FP64_DATA)[0]
+            ref[i, t] = float(np.dot(col.ravel(), m.ravel().astype(np.float64)))
+    geom = plan.geometry(torch.from_numpy(sv.coords).to(cuda))
+    out = plan.demigrate(torch.from_numpy(m).to(cuda), geom).cpu().numpy().astype(np.float64)
+    err = np.linalg.norm(out - ref) / np.linalg.norm(ref)
+    print(f"case: L vs oracle transpose, nh {nh}, jobs_x {jobs_x}: {np.count_nonzero(ref)} "
+          f"nonzero samples, relL2 {err:.2e}", flush=True)
+    assert np.count_nonzero(ref) > sv.ntr * sv.ns // 4
+    assert err <= 1e-6
+    assert np.max(np.abs(out - ref)) <= 1e-6 * np.max(np.abs(ref))
+
+
+def test_demigration_deterministic_and_scale_exact(cuda):
+    """L sums in 64-bit fixed point with a scale from max |km|: repeated
+    calls give the same bits, and scaling the image by 2^+-40 scales the
+    traces by exactly that."""
+    sv = surveys.line2d(nshots=6, ncols=240, ns=501, nscat=4, jobs_x=3)
+    plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
+    geom = plan.geometry(torch.from_numpy(sv.coords).to(cuda))
+    g = torch.Generator(device=cuda).manual_seed(23)
+    m = torch.randn(plan.image_shape, generator=g, device=cuda)
+    a = plan.demigrate(m, geom)
+    print(f"case: L repeatability and 2^+-40 scaling, {sv.ntr} traces", flush=True)
+    assert torch.equal(plan.demigrate(m, geom), a)
+    assert float(a.abs().max()) > 0.0
+    for s in (2.0 ** 40, 2.0 ** -40):
+        assert torch.equal(plan.demigrate(m * s, geom), a * s)
