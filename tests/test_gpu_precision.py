@@ -122,3 +122,27 @@ def test_quantized_storage_matches_numpy_error_feedback(cuda, dtype):
         want[:, t] = h
         e = (v - dec).astype(np.float32)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("nh,jobs_x", [(1, 1), (4, 3)])
+def test_compensated_line_vs_fp64(cuda, nh, jobs_x):
+    """The compensated migration of a 2-D line (8400 traces in one block,
+    400 columns; four offset classes and three job slabs in the second
+    case): the contribution count and the fold equal the oracle's exactly
+    and the image is within 1e-6 of the FP64 oracle."""
+    sv = surveys.line2d(nshots=70, ns=501, ncols=400, nscat=8, nh=nh, dh=1500.0 / nh,
+                        jobs_x=jobs_x, t0_range=(0.3, 1.8))
+    tr = surveys.traces_numpy(sv)
+    km64, fold64, n64 = oracle.migrate(sv.cfg, sv.ns, sv.dt, tr, sv.coords, sv.vel,
+                                       mode=oracle.FP64_DATA)
+    print(f"case: compensated 2-D line, {sv.ntr} traces, 400 columns, nh {nh}, "
+          f"jobs_x {jobs_x}", flush=True)
+    plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
+    km, fold, n = plan.migrate(torch.from_numpy(tr).to(cuda), torch.from_numpy(sv.coords).to(cuda),
+                               mode=PREP_CLOSED, accum=ACCUM_KAHAN, block=sv.ntr)
+    km = km.cpu().numpy()
+    r = rel_l2(km, km64)
+    print(f"  relL2 {r:.3e}")
+    assert n == n64
+    assert np.array_equal(fold.cpu().numpy(), fold64.astype(np.float32))
+    assert r <= 1e-6
