@@ -215,7 +215,13 @@ int ktm_poststack(const ktm_plan *plan, const float *km, const float *fold,
  *                   index, gates and filter, demigration / modelling)
  * and the post-stack chain image = S km (ktm_poststack, S linear for a fixed
  * fold) has the transpose km = S^T image (ktm_poststack_adjoint).  No
- * counterpart in the reference, which only migrates. */
+ * counterpart in the reference, which only migrates.
+ * ktm_demigrate sums each trace in 64-bit fixed point scaled from max |km|
+ * (a finite image): the result is deterministic and exact under scaling of
+ * km by powers of two; its quantum is max |km| * 2^(lg - 59), 2^lg >=
+ * columns x ns.  It sorts the traces by (offset class, CMP cell) into
+ * plan-owned buffers that grow on demand (a first call, or a larger ntr,
+ * allocates). */
 int ktm_demigrate(const ktm_plan *plan, const float *km, const ktm_trace_geom *geom,
                   int64_t ntr, float *traces, void *stream);
 int ktm_poststack_adjoint(const ktm_plan *plan, const float *image, const float *fold,
