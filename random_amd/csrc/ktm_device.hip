@@ -1854,10 +1854,21 @@ static mig_fn pick_accum(int accum, int *B)
 
 /* W waves share one image column (each owns K*64 consecutive samples);
  * a workgroup is 8 waves, so it covers 8/W columns. */
-static mig_fn pick_migrate(int ns, int accum, int *W, int *K, int *B)
+static mig_fn pick_migrate(int ns, int accum, int64_t wgs8, int *W, int *K, int *B)
 {
-    if (ns <= 512) { *W = 1; *K = 8; return pick_accum<8, 1>(accum, B); }
-    if (ns <= 1024) { *W = 2; *K = 8; return pick_accum<8, 2>(accum, B); }
+    /* wgs8: workgroups of 8 columns (x offset classes) in the region.  An
+     * image too small to give every CU two workgroups at the default layout
+     * takes twice the waves per column with half the samples per lane: on
+     * the 800-column, 1001-sample line 3.77 ms against 4.79 ms (the 1536-
+     * sample layout gained nothing from 6 x 4 on a 1600-column line). */
+    if (ns <= 512) {
+        if (wgs8 < 512) { *W = 2; *K = 4; return pick_accum<4, 2>(accum, B); }
+        *W = 1; *K = 8; return pick_accum<8, 1>(accum, B);
+    }
+    if (ns <= 1024) {
+        if (2 * wgs8 < 512) { *W = 4; *K = 4; return pick_accum<4, 4>(accum, B); }
+        *W = 2; *K = 8; return pick_accum<8, 2>(accum, B);
+    }
     /* (the Kahan K = 12 variant runs at 0.96x the plain one; four waves of
      * K = 6 per column measured 0.72x, of K = 8 0.63x, before the Kahan sums
      * were blocked per batch) */
@@ -1943,7 +1954,8 @@ extern "C" int ktm_migrate_block(const ktm_plan *p, const void *prepared, int dt
         a.ry0 = (job % p->cfg.jobs_y) * s->ny; a.ry1 = a.ry0 + s->ny;
     }
     int W = 1, K = 8, B = 8;
-    mig_fn fn = pick_migrate(s->ns, accum, &W, &K, &B);
+    const int64_t ncols_region = (int64_t)(a.rx1 - a.rx0) * (a.ry1 - a.ry0);
+    mig_fn fn = pick_migrate(s->ns, accum, (ncols_region + 7) / 8 * s->nh, &W, &K, &B);
     /* tile of 8/W columns: a row on a 2-D line, as square as possible in 3-D */
     const int ncol = 8 / W;
     a.tx_cols = s->nyp == 1 ? ncol : (ncol == 8 ? 4 : ncol == 4 ? 2 : ncol);

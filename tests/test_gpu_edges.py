@@ -18,7 +18,10 @@ pytestmark = pytest.mark.gpu
 
 # sample counts at and around the lane-layout boundaries of ktm_migrate_block
 # (K samples per lane x W waves per column: 8x1 <= 512 < 8x2 <= 1024 < 12x2
-# <= 1536 < 8x4 <= 2048 < 12x4 <= 3072 < 8x8 <= 4096 < 12x8 <= 6144)
+# <= 1536 < 8x4 <= 2048 < 12x4 <= 3072 < 8x8 <= 4096 < 12x8 <= 6144; images
+# too small for two workgroups per CU take 4x2 and 4x4 below 1025 samples,
+# as the 40-column line here does -- test_wide_image_layouts_bitexact keeps
+# the default ones covered)
 NS_CASES = [4, 63, 512, 513, 1024, 1025, 1536, 1537, 2048, 2049, 3072, 3073, 4096, 4097, 6144]
 
 
@@ -53,6 +56,17 @@ def test_every_kernel_variant_bitexact(cuda, ns):
     sv = small_line(ns)
     tr = surveys.traces_numpy(sv)
     assert np.abs(tr).max() > 0
+    _, km, _, n = bitexact_case(sv, tr, cuda, block=25)
+    assert n > 0 and np.abs(km).max() > 0
+
+
+@pytest.mark.parametrize("ns", [63, 512, 513, 1024])
+def test_wide_image_layouts_bitexact(cuda, ns):
+    """A 4104-column line (more than two workgroups per CU at 8 samples per
+    lane) keeps the default layouts below 1025 samples."""
+    print(f"case: wide-image variant at ns={ns}, 60 traces, 4104 columns, 2 jobs", flush=True)
+    sv = small_line(ns, ncols=4104)
+    tr = surveys.traces_numpy(sv)
     _, km, _, n = bitexact_case(sv, tr, cuda, block=25)
     assert n > 0 and np.abs(km).max() > 0
 
