@@ -19,13 +19,14 @@
  *   J         = sum_{X,t} sum_{h < nh-1} (I[X][h+1][t] - I[X][h][t])^2.
  *
  * k_vgrad: one workgroup owns one image column X and one time chunk of
- * `chunk` (<= 256) samples, one lane per sample.  Pass 1 accumulates all nh
- * offset-class samples of the chunk (Kahan) in LDS, forms the chunk's share
- * of J (FP64) and dJ/dI in place; pass 2 walks the same traces again and
- * recomputes each traveltime to accumulate
+ * `chunk` (<= 256) samples, one lane per sample.  The offset classes stream
+ * in order: pass 1 sums class h+1 (Kahan) while the lane holds I[h-1..h]
+ * of its sample in registers, adds its share of J (FP64), and with dJ/dI[h]
+ * known pass 2 walks class h's traces again and recomputes each traveltime
+ * to accumulate
  *   dJ/dv(X,t) = sum_h dJ/dI[h][t] sum_c F'(q_c) dq_c/dv.
- * Nothing per contribution is stored: the time chunk bounds the working set
- * to the LDS of one workgroup.  Traces are binned on the host by (offset
+ * Nothing per contribution is stored: the time chunk and a window of three
+ * classes bound the working set to registers.  Traces are binned on the host by (offset
  * class, CMP inline coordinate), so a column's traces of one class are a
  * contiguous range; every wave walks the ranges itself (gates lane-parallel,
  * ballot, then the passing traces in order) and needs no barrier.
@@ -42,7 +43,11 @@
 
 #define VG_NT 256
 #define VG_WAVE 64
-#define VG_G 16         /* passing traces whose taps are loaded together */
+/* passing traces whose taps are loaded together: with the classes streamed
+ * (no LDS) 2 keeps the kernel at 38 VGPRs and 8 waves per SIMD, which hide
+ * the L2 latency of the taps; 1 to 4 measured the same, 8 (107 VGPRs) 12 %
+ * and 16 (164 VGPRs, 3 waves) 23 % slower */
+#define VG_G 2
 
 struct ktm_vgrad {
     ktm_config cfg;
@@ -200,9 +205,8 @@ __device__ __forceinline__ float vg_class_pass(const VgArgs &a, int lo, int hi, 
         uint64_t m = __ballot(pass);
         while (m) {
             /* up to VG_G passing traces at a time, in order: their taps are
-             * loaded together so that the loads overlap (a lone trace's four
-             * loads would leave the wave waiting on L2 latency); slots past
-             * the last passing trace repeat the first and are not added */
+             * loaded together so that the loads overlap; slots past the last
+             * passing trace repeat the first and are not added */
             int jc[VG_G], nb = 0;
 #pragma unroll
             for (int c = 0; c < VG_G; ++c) {
@@ -257,9 +261,7 @@ __device__ __forceinline__ float vg_class_pass(const VgArgs &a, int lo, int hi, 
 
 __global__ void __launch_bounds__(VG_NT) k_vgrad(const VgArgs a)
 {
-    extern __shared__ double vg_lds[];
-    double *sRed = vg_lds;                               /* [VG_NT / 64] */
-    float *sI = reinterpret_cast<float *>(vg_lds + VG_NT / VG_WAVE);   /* [nh][VG_NT] */
+    __shared__ double sRed[VG_NT / VG_WAVE];
     const int tid = threadIdx.x, lane = tid & (VG_WAVE - 1), wv = tid / VG_WAVE;
     /* XCD-aware order: consecutive workgroup ids go to different XCDs; each
      * XCD takes runs of a.run consecutive (column, chunk) tiles (neighbouring
@@ -283,21 +285,31 @@ __global__ void __launch_bounds__(VG_NT) k_vgrad(const VgArgs a)
     }
     const int32_t *rg = a.ranges + (int64_t)X * a.nh * 2;
 
-    /* pass 1: the chunk of every offset-class trace of the gather */
-    for (int h = 0; h < a.nh; ++h) {
-        const float s = vg_class_pass<true>(a, rg[2 * h], rg[2 * h + 1], IX, r, th, t2, t, lane);
-        sI[h * VG_NT + tid] = act ? s : 0.0f;
-    }
-    /* objective share and dJ/dI (own lane's column of sI only) */
+    /* The offset classes stream in order: the lane holds I[h-1..h+1] of its
+     * sample, the objective share adds (I[h+1] - I[h])^2, and as soon as
+     * dJ/dI[h] = 2 ((I[h] - I[h-1]) - (I[h+1] - I[h])) is known the second
+     * pass recomputes class h's traveltimes for dJ/dv.  No per-class array,
+     * so no LDS bounds the occupancy. */
     double jl = 0.0;
-    float dprev = 0.0f;
+    float dprev = 0.0f, g = 0.0f;
+    float Ih = vg_class_pass<true>(a, rg[0], rg[1], IX, r, th, t2, t, lane);
+    if (!act) Ih = 0.0f;
     for (int h = 0; h < a.nh; ++h) {
-        const float Ih = sI[h * VG_NT + tid];
-        const float dn = h + 1 < a.nh ? sI[(h + 1) * VG_NT + tid] - Ih : 0.0f;
+        float In = 0.0f, dn = 0.0f;
+        if (h + 1 < a.nh) {
+            In = vg_class_pass<true>(a, rg[2 * h + 2], rg[2 * h + 3], IX, r, th, t2, t, lane);
+            if (!act) In = 0.0f;
+            dn = In - Ih;
+        }
         jl += (double)dn * (double)dn;
         if (a.cig && act) a.cig[((int64_t)X * a.nh + h) * a.ns + t] = Ih;
-        sI[h * VG_NT + tid] = 2.0f * (dprev - dn);
+        if (a.gv) {
+            const float rh = 2.0f * (dprev - dn);
+            const float S = vg_class_pass<false>(a, rg[2 * h], rg[2 * h + 1], IX, r, th, t2, t, lane);
+            g = __builtin_fmaf(rh, S, g);
+        }
         dprev = dn;
+        Ih = In;
     }
     for (int o = VG_WAVE / 2; o > 0; o >>= 1) jl += __shfl_down(jl, o);
     if (lane == 0) sRed[wv] = jl;
@@ -307,26 +319,12 @@ __global__ void __launch_bounds__(VG_NT) k_vgrad(const VgArgs a)
         for (int w = 0; w < VG_NT / VG_WAVE; ++w) s += sRed[w];
         a.part[tile] = s;
     }
-    if (!a.gv) return;
-
-    /* pass 2: recompute every traveltime for dJ/dv */
-    float g = 0.0f;
-    for (int h = 0; h < a.nh; ++h) {
-        const float rh = sI[h * VG_NT + tid];
-        const float S = vg_class_pass<false>(a, rg[2 * h], rg[2 * h + 1], IX, r, th, t2, t, lane);
-        g = __builtin_fmaf(rh, S, g);
-    }
-    if (act) a.gv[(int64_t)X * a.ns + t] = kfac * g;
+    if (a.gv && act) a.gv[(int64_t)X * a.ns + t] = kfac * g;
 }
 
 /* ------------------------------------------------------------------------ */
 /* host side                                                                 */
 /* ------------------------------------------------------------------------ */
-static size_t vg_lds_bytes(int nh)
-{
-    return sizeof(double) * (VG_NT / VG_WAVE) + sizeof(float) * (size_t)nh * VG_NT;
-}
-
 extern "C" int ktm_vgrad_destroy(ktm_vgrad *p)
 {
     if (!p) return KTM_OK;
@@ -353,7 +351,6 @@ extern "C" int ktm_vgrad_create(ktm_vgrad **out, const ktm_config *cfg, int ns, 
     if (st != KTM_OK) { free(p); return st; }
     const ktm_setup *s = &p->tb.s;
     if (s->nyp != 1 || s->nxp < 2) { ktm_vgrad_destroy(p); return KTM_ERR_UNSUPPORTED; }
-    if (vg_lds_bytes(s->nh) > 160 * 1024) { ktm_vgrad_destroy(p); return KTM_ERR_UNSUPPORTED; }
     p->cfg = *cfg;
     p->ns = ns; p->dt = dt; p->nxp = s->nxp; p->nh = s->nh;
     p->nt_nodes = nt_nodes; p->nx_nodes = nx_nodes; p->chunk = chunk;
@@ -527,10 +524,7 @@ extern "C" int ktm_vgrad_eval(ktm_vgrad *p, const float *V, double *J, double *d
     if (a.run < 1) a.run = 1;
     const int64_t span = 8 * (int64_t)a.run;
     const int64_t nblk = ((int64_t)ntiles + span - 1) / span * span;
-    const size_t lds = vg_lds_bytes(p->nh);
-    VG_TRY(hipFuncSetAttribute((const void *)k_vgrad, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds));
-    hipLaunchKernelGGL(k_vgrad, dim3((unsigned)nblk), dim3(VG_NT), lds, st, a);
+    hipLaunchKernelGGL(k_vgrad, dim3((unsigned)nblk), dim3(VG_NT), 0, st, a);
     VG_TRY(hipGetLastError());
     if (dJdV) {
         const int64_t n1 = (int64_t)p->nxp * p->nt_nodes;
