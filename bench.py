@@ -54,14 +54,14 @@ def parse():
     ap.add_argument("--ncols", type=int, default=400)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline time")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the ingest, velocity-gradient, adjoint/LSM and EOM legs")
+    ap.add_argument("--no-extra", action="store_true", help="skip the ingest, velocity-gradient, adjoint/LSM, post-stack and EOM legs")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3")
     ap.add_argument("--leg-steps", type=int, default=3, help="timed blocks per precision leg")
     ap.add_argument("--err-traces", type=int, default=20000,
                     help="traces of the FP64-oracle error check on the 50x50x1500 sub-volume")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy 3x (HBM counter calibration) and exit")
-    ap.add_argument("--leg", choices=("vgrad", "adjoint", "eom"),
+    ap.add_argument("--leg", choices=("vgrad", "adjoint", "poststack", "eom"),
                     help="only run this extra leg, print its JSON and exit (for profiling)")
     return ap.parse_args()
 
@@ -346,6 +346,61 @@ def adjoint_leg(dev, reps=3, niter=20):
             "correlation_with_truth": {"migration": round(corr0, 4), "lsm": round(corr, 4)}}
 
 
+def poststack_leg(dev, plan=None, reps=3):
+    """The steps after the migration on the 3-D image (400 x 400 x 1500,
+    one job slab): the post-stack filter of Kirchhoff.c:1423-1479 (derivative,
+    fold, low-pass as an FP64 circular convolution) and the image<job>.su
+    trace records (big-endian headers and samples, Kirchhoff.c:1437-1598),
+    on a seeded image and fold."""
+    import numpy as np
+    import torch
+
+    from random_amd import _lib, surveys
+    from random_amd.ktm import Plan
+    if plan is None:
+        sv = surveys.area3d(ntr=1000)
+        plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
+    g = torch.Generator(device=dev).manual_seed(7)
+    km = torch.randn(plan.image_shape, generator=g, device=dev)
+    fold = torch.randint(1, 200, plan.image_shape, generator=g, device=dev).float()
+    nx, ny, nh, ns = plan.setup["nx"], plan.setup["ny"], plan.nh, plan.ns
+    nrec = nx * ny * nh
+    rec = torch.empty(nrec * (240 + 4 * ns), dtype=torch.uint8, device=dev)
+    L = _lib.lib()
+    stream = torch.cuda.current_stream().cuda_stream
+    img = plan.poststack(km, fold)
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    ms_p, ms_r = [], []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        e0.record()
+        img = plan.poststack(km, fold)
+        e1.record()
+        _lib.check(L.ktm_image_records(plan._h, img.data_ptr(), 0, rec.data_ptr(), stream),
+                   "ktm_image_records")
+        e2.record()
+        torch.cuda.synchronize()
+        ms_p.append(e0.elapsed_time(e1))
+        ms_r.append(e1.elapsed_time(e2))
+    ms_p, ms_r = float(np.median(ms_p)), float(np.median(ms_r))
+    ntraces = int(np.prod(plan.image_shape[:3]))
+    flop = 2.0 * ntraces * ns * ns
+    bytes_p = 3 * 4 * ntraces * ns            # km and fold in, image out
+    bytes_r = 4 * nrec * ns + rec.numel()     # image in, records out
+    log(f"poststack: {ms_p:.2f} ms ({flop / ms_p / 1e9:.1f} TFLOP/s FP64); records {ms_r:.2f} ms "
+        f"({bytes_r / ms_r / 1e6:.0f} GB/s)")
+    return {"workload": f"3-D image {plan.image_shape[0]} x {plan.image_shape[1]} x {ns}, "
+                        f"{ntraces} image traces",
+            "ms_poststack": round(ms_p, 2),
+            "poststack_TFLOP_per_s_fp64": round(flop / (ms_p * 1e-3) / 1e12, 1),
+            "poststack_fp64_peak_frac": round(flop / (ms_p * 1e-3) / 78.6e12, 3),
+            "poststack_GB_per_s": round(bytes_p / (ms_p * 1e-3) / 1e9, 1),
+            "ms_image_records": round(ms_r, 2),
+            "image_records_bytes": int(bytes_r),
+            "image_records_GB_per_s": round(bytes_r / (ms_r * 1e-3) / 1e9, 1),
+            "image_records_hbm_frac": round(bytes_r / (ms_r * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)}
+
+
 def eom_leg(dev, reps=3, cpu_traces=48):
     """Equivalent-offset migration (EOM.c csp()) of the configuration-0 line:
     4800 traces x 1001 samples into 800 CSPs x 160 offset bins, plus the post
@@ -398,7 +453,8 @@ def main():
         return
     if args.leg:
         import torch
-        fn = {"vgrad": vgrad_leg, "adjoint": adjoint_leg, "eom": eom_leg}[args.leg]
+        fn = {"vgrad": vgrad_leg, "adjoint": adjoint_leg, "poststack": poststack_leg,
+              "eom": eom_leg}[args.leg]
         print(json.dumps(fn(torch.device("cuda:0"))), flush=True)
         return
     if args.gpus != 1:
@@ -528,6 +584,7 @@ def main():
         result["ingest"] = ingest_leg(sv, plan, traces, args.block, dev)
         result["velocity_gradient"] = vgrad_leg(dev)
         result["adjoint_lsm"] = adjoint_leg(dev)
+        result["poststack"] = poststack_leg(dev, plan)
         result["eom"] = eom_leg(dev)
     if not args.no_cpu and not args.profile:
         nthreads = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))

@@ -1763,20 +1763,36 @@ __global__ void k_image_headers(const ImgHdrArgs a, uint32_t *__restrict__ rec)
 }
 
 /* samples of image<job>.su: record r of the slab <- image[X][Y][ih][t] */
-__global__ void k_image_samples(const float *__restrict__ image, const ImgHdrArgs a,
-                                uint32_t *__restrict__ rec)
+/* the samples of image<job>.su: one workgroup per record r, its image row
+ * byte-swapped into place; 16-byte vectors when ns % 4 == 0 (image rows and
+ * records then start 16-byte aligned: 240-byte header, 4 (60 + ns)-byte
+ * records) */
+template <bool V4>
+__global__ void __launch_bounds__(128) k_image_samples(const float *__restrict__ image,
+                                                        const ImgHdrArgs a,
+                                                        uint32_t *__restrict__ rec)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t nrec = (int64_t)a.nx * a.ny * a.nh;
-    if (e >= nrec * a.ns) return;
-    const int64_t r = e / a.ns;
-    const int t = (int)(e - r * a.ns);
+    const int64_t r = blockIdx.x;
     const int ih = (int)(r % a.nh);
     const int64_t c = r / a.nh;
     const int64_t X = c / a.ny + (int64_t)(a.job / a.jobs_y) * a.nx;
     const int64_t Y = c % a.ny + (int64_t)(a.job % a.jobs_y) * a.ny;
-    const float v = image[((X * a.nyp + Y) * a.nh + ih) * a.ns + t];
-    rec[r * (60 + (int64_t)a.ns) + 60 + t] = __builtin_bswap32(__float_as_uint(v));
+    const float *src = image + ((X * a.nyp + Y) * a.nh + ih) * a.ns;
+    uint32_t *dst = rec + r * (60 + (int64_t)a.ns) + 60;
+    if (V4) {
+        const int n4 = a.ns / 4;
+        for (int i = threadIdx.x; i < n4; i += blockDim.x) {
+            uint4 w = reinterpret_cast<const uint4 *>(src)[i];
+            w.x = __builtin_bswap32(w.x);
+            w.y = __builtin_bswap32(w.y);
+            w.z = __builtin_bswap32(w.z);
+            w.w = __builtin_bswap32(w.w);
+            reinterpret_cast<uint4 *>(dst)[i] = w;
+        }
+    } else {
+        for (int t = threadIdx.x; t < a.ns; t += blockDim.x)
+            dst[t] = __builtin_bswap32(__float_as_uint(src[t]));
+    }
 }
 
 extern "C" int ktm_segy_encode(const float *traces, int64_t ntr, int ns, int format,
@@ -1820,14 +1836,18 @@ extern "C" int ktm_image_records(const ktm_plan *p, const float *image, int job,
     a.job = job; a.jobs_y = c->jobs_y;
     a.dt_us = (uint32_t)(uint16_t)(st->dt * 1000000);
     const int64_t nrec = (int64_t)a.nx * a.ny * a.nh;
-    const int64_t n = nrec * a.ns;
     const int nt = 256;
-    if ((n + nt - 1) / nt > 0x7fffffffLL) return KTM_ERR_UNSUPPORTED;
+    if (nrec > 0x7fffffffLL) return KTM_ERR_UNSUPPORTED;
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_image_headers, dim3((unsigned)((nrec + nt - 1) / nt)), dim3(nt), 0, s, a,
                        (uint32_t *)records);
-    hipLaunchKernelGGL(k_image_samples, dim3((unsigned)((n + nt - 1) / nt)), dim3(nt), 0, s, image,
-                       a, (uint32_t *)records);
+    const bool v4 = a.ns % 4 == 0 && ((uintptr_t)records & 15u) == 0 && ((uintptr_t)image & 15u) == 0;
+    if (v4)
+        hipLaunchKernelGGL(k_image_samples<true>, dim3((unsigned)nrec), dim3(128), 0, s, image, a,
+                           (uint32_t *)records);
+    else
+        hipLaunchKernelGGL(k_image_samples<false>, dim3((unsigned)nrec), dim3(128), 0, s, image, a,
+                           (uint32_t *)records);
     HIP_TRY(hipGetLastError());
     return KTM_OK;
 }

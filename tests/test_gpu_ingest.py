@@ -145,9 +145,12 @@ def test_encode_matches_host_writer(cuda, tmp_path, ieee):
         assert rel.max() < 2.0 ** -20
 
 
-def test_image_records_match_host_headers(cuda):
-    print("case: image records vs host headers", flush=True)
-    sv = surveys.area3d(ntr=300, ns=257, ncols=12, extent=600.0, max_offset=900.0, nscat=2,
+@pytest.mark.parametrize("ns", [257, 256])
+def test_image_records_match_host_headers(cuda, ns):
+    """Both sample writers: word by word (ns = 257) and 16-byte vectors
+    (ns % 4 == 0)."""
+    print(f"case: image records vs host headers, ns {ns}", flush=True)
+    sv = surveys.area3d(ntr=300, ns=ns, ncols=12, extent=600.0, max_offset=900.0, nscat=2,
                         jobs_x=2, jobs_y=3)
     cfg = sv.cfg.replace(angle_e_to_n=-10.369, ox=444479.29, oy=478003.98, nh=3, dh=300.0,
                          oh=50.0, start_iline=1500, end_iline=1512, start_xline=2500,
