@@ -16,8 +16,8 @@
  *                   tile of image columns (one wave per column, all ns samples
  *                   in registers), streams the traces of a block through LDS in
  *                   trace order and accumulates F(floor(dsr/dt)) per sample
- *   k_poststack_mx  derivative/fold + low-pass, Kirchhoff.c:1457-1479, as an
- *                   FP64 matrix product on the matrix cores
+ *   k_postmx<PostKtm>  derivative/fold + low-pass, Kirchhoff.c:1457-1479, as
+ *                   an FP64 matrix product on the matrix cores (ktm_postmx.h)
  *   k_image_*       image<job>.su trace records (headers + big-endian
  *                   samples), Kirchhoff.c:1437-1598
  *   k_demigrate     exact transpose of k_migrate (CLOSED prep)
@@ -37,6 +37,7 @@
 
 #include "ktm_internal.h"
 #include "ktm_devmath.h"
+#include "ktm_postmx.h"
 
 #define KTM_WAVE 64
 
@@ -855,83 +856,14 @@ __global__ void k_image_reset(float *km, float *fold, float *comp, int64_t n)
  * m < 2 ns.  PS_NT: threads per workgroup of the transpose. */
 #define PS_NT 256
 
-/* The post-stack low-pass as an FP64 matrix product on the matrix cores:
- * out[t][j] = sum_u H[t][u] d[u][j], H[t][u] = h[(t - u) mod ns] (the
- * circulant of the even kernel h, h2[m] = h[m mod ns] for m < 2 ns), d the
- * derivative traces of Kirchhoff.c:1457-1462.  A workgroup = 16 image traces
- * (the 16 columns of the MFMA tiles) x up to PM_WAVES * NTW * 16 samples
- * (gridDim.y groups cover longer traces); wave w owns NTW 16-row tiles.
- * d streams through LDS in chunks of PM_UC samples; each v_mfma_f64_16x16x4
- * takes the chunk's 4 x 16 block of d (one LDS read shared by the wave's NTW
- * tiles) and a 16 x 4 block of H read from h.  Operand layout of
- * v_mfma_f64_16x16x4f64 for lane l (measured with one-hot operands):
- * A[l % 16][l / 16], B[l / 16][l % 16], D[4 r + l / 16][l % 16] in result
- * register r. */
-#define PM_WAVES 8
-#define PM_UC 256                    /* samples of d per LDS chunk */
-#define PM_LD 17                     /* padded row of the chunk (16 traces) */
-typedef double pm_d4 __attribute__((ext_vector_type(4)));
-
-template <int NTW, bool HG>
-__global__ void __launch_bounds__(PM_WAVES * KTM_WAVE)
-k_poststack_mx(const float *__restrict__ km, const float *__restrict__ fold, int64_t ntraces,
-               int ns, float dt, const double *__restrict__ h2, float *__restrict__ out)
-{
-    extern __shared__ double pm_lds[];
-    double *zs = pm_lds;                               /* [PM_UC][PM_LD] */
-    double *hl = pm_lds + PM_UC * PM_LD;               /* [2 ns] when !HG */
-    const double *hh = HG ? h2 : hl;
-    const int tid = threadIdx.x, lane = tid & (KTM_WAVE - 1), wv = tid / KTM_WAVE;
-    const int64_t tr0 = (int64_t)blockIdx.x * 16;
-    const int tw0 = (blockIdx.y * PM_WAVES + wv) * NTW * 16;   /* first row of the wave */
-    if (!HG)
-        for (int m = tid; m < 2 * ns; m += PM_WAVES * KTM_WAVE) hl[m] = h2[m];
-    pm_d4 acc[NTW];
-#pragma unroll
-    for (int i = 0; i < NTW; ++i) acc[i] = (pm_d4){0.0, 0.0, 0.0, 0.0};
-    const int ar = lane & 15, ak = lane >> 4;          /* A row, k of this lane */
-    const int n2 = 2 * ns - 1;
-    for (int c0 = 0; c0 < ns; c0 += PM_UC) {
-        __syncthreads();                               /* previous chunk consumed */
-        /* d[u][j] of the chunk, Kirchhoff.c:1460 (a float expression stored
-         * into a double), 0 for u = 0, past the trace or past the image */
-        for (int e = tid; e < PM_UC * 16; e += PM_WAVES * KTM_WAVE) {
-            const int j = e / PM_UC, ul = e - j * PM_UC, u = c0 + ul;
-            const int64_t tr = tr0 + j;
-            double v = 0.0;
-            if (tr < ntraces && u >= 1 && u < ns) {
-                const float *k = km + tr * ns;
-                v = (double)(-dt * (float)u * (k[u] - k[u - 1]) * dt / fold[tr * ns + u]);
-            }
-            zs[ul * PM_LD + j] = v;
-        }
-        __syncthreads();
-        const int nu = min(PM_UC, ((ns - c0) + 3) & ~3);
-        for (int k0 = 0; k0 < nu; k0 += 4) {
-            const double b = zs[(k0 + ak) * PM_LD + ar];  /* B[ak][ar] = d[c0+k0+ak][j=ar] */
-            const int u = c0 + k0 + ak;
-#pragma unroll
-            for (int i = 0; i < NTW; ++i) {
-                /* A[ar][ak] = H[t][u] = h2[t - u + ns], clamped into h2 (rows
-                 * past ns are not stored; columns past ns meet d = 0) */
-                const int m = min(max(tw0 + i * 16 + ar - u + ns, 0), n2);
-                acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(hh[m], b, acc[i], 0, 0, 0);
-            }
-        }
+/* x of the post-stack filter, Kirchhoff.c:1460: a float expression stored
+ * into a double */
+struct PostKtm {
+    static __device__ __forceinline__ double x(const float *k, const float *f, int u, float dt)
+    {
+        return (double)(-dt * (float)u * (k[u] - k[u - 1]) * dt / f[u]);
     }
-    /* D[4 r + l / 16][l % 16]: rows t, column j = trace */
-    const int64_t tr = tr0 + ar;
-    if (tr < ntraces) {
-#pragma unroll
-        for (int i = 0; i < NTW; ++i) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int t = tw0 + i * 16 + 4 * r + ak;
-                if (t < ns) out[tr * ns + t] = t == 0 ? 0.0f : (float)acc[i][r];
-            }
-        }
-    }
-}
+};
 
 /* ------------------------------------------------------------------------ */
 /* seeded synthetic surveys                                                  */
@@ -2086,29 +2018,9 @@ extern "C" int ktm_poststack(const ktm_plan *p, const float *km, const float *fo
 {
     if (!p || !km || !fold || !out) return KTM_ERR_ARG;
     const ktm_setup *s = &p->tb.s;
-    const int ns = s->ns;
-    const int64_t ntraces = (int64_t)s->nxp * s->nyp * s->nh;
-    typedef void (*fn_t)(const float *, const float *, int64_t, int, float, const double *, float *);
-    /* rows per wave: the smallest of 4, 8, 12 tiles of 16 covering ns over
-     * the 8 waves, else 12 and several row groups */
-    const int ntw = ns <= PM_WAVES * 4 * 16 ? 4 : ns <= PM_WAVES * 8 * 16 ? 8 : 12;
-    const int rows = PM_WAVES * ntw * 16;
-    const int groups = (ns + rows - 1) / rows;
-    /* h in LDS when it fits beside the d chunk, else read through L1/L2 */
-    const size_t lds_z = sizeof(double) * PM_UC * PM_LD;
-    const bool hg = lds_z + sizeof(double) * 2 * (size_t)ns > 150 * 1024;
-    const size_t lds = lds_z + (hg ? 0 : sizeof(double) * 2 * (size_t)ns);
-    fn_t fn;
-    if (ntw == 4) fn = hg ? k_poststack_mx<4, true> : k_poststack_mx<4, false>;
-    else if (ntw == 8) fn = hg ? k_poststack_mx<8, true> : k_poststack_mx<8, false>;
-    else fn = hg ? k_poststack_mx<12, true> : k_poststack_mx<12, false>;
-    const int64_t nblk = (ntraces + 15) / 16;
-    if (nblk > 0x7fffffffLL || groups > 65535) return KTM_ERR_UNSUPPORTED;
-    HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(fn, dim3((unsigned)nblk, (unsigned)groups), dim3(PM_WAVES * KTM_WAVE), lds,
-                       (hipStream_t)stream, km, fold, ntraces, ns, s->dt, p->d_hconv2, out);
-    HIP_TRY(hipGetLastError());
-    return KTM_OK;
+    const hipError_t e = postmx_launch<PostKtm>(km, fold, (int64_t)s->nxp * s->nyp * s->nh, s->ns,
+                                                s->dt, p->d_hconv2, out, (hipStream_t)stream);
+    return e == hipSuccess ? KTM_OK : e == hipErrorInvalidValue ? KTM_ERR_UNSUPPORTED : KTM_ERR_DEVICE;
 }
 
 extern "C" int ktm_device_alloc(void **ptr, size_t bytes)

@@ -21,9 +21,10 @@
  *   latest trace stay in registers until another trace needs their slot.
  *   Traces stream in chunks of 256: the aperture gate of each is evaluated
  *   lane-parallel, the passing ones are compacted in order through LDS.
- * k_eom_post: the post filter as a real even-kernel circular convolution in
- *   FP64 (the (1+i) factor of the sqrt-rho spectrum leaves the real part of a
- *   Hermitian product unchanged), g = IDFT(lowpass * rootrho) / ns.
+ * k_postmx<PostEom> (ktm_postmx.h): the post filter as a real even-kernel
+ *   circular convolution in FP64 on the matrix cores (the (1+i) factor of
+ *   the sqrt-rho spectrum leaves the real part of a Hermitian product
+ *   unchanged), g = IDFT(lowpass * rootrho) / ns.
  * k_eom_records: the bytes of image<job>.su (EOM.c:1346-1537), one thread
  *   per 32-bit word.
  * The gathers are either the whole padded volume [nxp][nyp][nh][ns] or, for
@@ -39,6 +40,7 @@
 
 #include "ktm_internal.h"
 #include "ktm_devmath.h"
+#include "ktm_postmx.h"
 
 #define EOM_NT 256
 #define EOM_RUN 16     /* consecutive tiles per XCD run */
@@ -434,59 +436,14 @@ __global__ void k_eom_reset(float *csp, float *fold, int64_t n)
     }
 }
 
-/* EOM.c:1386-1413 for TR gather traces per workgroup: x = (csp/fold) dt t
- * (float), out = g (*) x in FP64, out[0] = 0.  g2[m] = g[m mod ns] (m < 2 ns)
- * is copied to LDS, or (HG, long traces) read from global memory. */
-template <int SLOTS, int TR, bool HG>
-__global__ void __launch_bounds__(EOM_NT)
-k_eom_post(const float *__restrict__ csp, const float *__restrict__ fold, int64_t ntraces, int ns,
-           float dt, const double *__restrict__ g2, float *__restrict__ out)
-{
-    extern __shared__ double eom_lds[];
-    double *xx = eom_lds;                          /* [ns][TR]              */
-    double *gl = eom_lds + (int64_t)ns * TR;       /* [2*ns] when !HG       */
-    const double *gg = HG ? g2 : gl;
-    const int64_t tr0 = (int64_t)blockIdx.x * TR;
-    if (!HG)
-        for (int m = threadIdx.x; m < 2 * ns; m += EOM_NT) gl[m] = g2[m];
-    for (int e = threadIdx.x; e < ns * TR; e += EOM_NT) {
-        const int u = e / TR, b = e - u * TR;
-        const int64_t tr = tr0 + b;
-        double v = 0.0;
-        if (tr < ntraces && u >= 1)
-            v = (double)((csp[tr * ns + u] / fold[tr * ns + u]) * dt * (float)u);   /* EOM.c:1391 */
-        xx[u * TR + b] = v;
+/* x of the post filter, EOM.c:1391: the float (csp / fold) dt t, stored into
+ * a double; out = g (*) x in FP64 (k_postmx), out[0] = 0 */
+struct PostEom {
+    static __device__ __forceinline__ double x(const float *c, const float *f, int u, float dt)
+    {
+        return (double)((c[u] / f[u]) * dt * (float)u);
     }
-    __syncthreads();
-    double acc[SLOTS][TR];
-#pragma unroll
-    for (int s = 0; s < SLOTS; ++s)
-#pragma unroll
-        for (int b = 0; b < TR; ++b) acc[s][b] = 0.0;
-    for (int u = 0; u < ns; ++u) {
-        double xv[TR];
-#pragma unroll
-        for (int b = 0; b < TR; ++b) xv[b] = xx[u * TR + b];
-#pragma unroll
-        for (int s = 0; s < SLOTS; ++s) {
-            const int t = s * EOM_NT + threadIdx.x;
-            const double gv = t < ns ? gg[t - u + ns] : 0.0;
-#pragma unroll
-            for (int b = 0; b < TR; ++b) acc[s][b] = __builtin_fma(gv, xv[b], acc[s][b]);
-        }
-    }
-#pragma unroll
-    for (int s = 0; s < SLOTS; ++s) {
-        const int t = s * EOM_NT + threadIdx.x;
-        if (t < ns) {
-#pragma unroll
-            for (int b = 0; b < TR; ++b) {
-                const int64_t tr = tr0 + b;
-                if (tr < ntraces) out[tr * ns + t] = t == 0 ? 0.0f : (float)acc[s][b];
-            }
-        }
-    }
-}
+};
 
 /* ------------------------------------------------------------------------ */
 /* entry points                                                              */
@@ -567,47 +524,17 @@ extern "C" int ktm_eom_job_block(const ktm_eom_plan *p, const float *traces,
     return eom_block(p, traces, coords, ntr, job, true, csp, fold, counter, stream);
 }
 
-typedef void (*eom_post_fn)(const float *, const float *, int64_t, int, float, const double *,
-                            float *);
-
-/* traces per workgroup and where g lives, so the FP64 arrays fit the LDS */
-template <int SLOTS>
-static bool eom_post_try(int ns, eom_post_fn *fn, int *tr, size_t *lds)
-{
-    const size_t cap = 160 * 1024 / sizeof(double), n = (size_t)ns;
-    if (6 * n <= cap) { *fn = k_eom_post<SLOTS, 4, false>; *tr = 4; *lds = 6 * n; }
-    else if (3 * n <= cap) { *fn = k_eom_post<SLOTS, 1, false>; *tr = 1; *lds = 3 * n; }
-    else if (n <= cap) { *fn = k_eom_post<SLOTS, 1, true>; *tr = 1; *lds = n; }
-    else return false;
-    *lds *= sizeof(double);
-    return true;
-}
 
 static int eom_post(const ktm_eom_plan *p, const float *csp, const float *fold, int64_t ntraces,
                     float *out, void *stream)
 {
     if (!p || !csp || !fold || !out) return KTM_ERR_ARG;
     const int ns = p->s.ns;
-    const int slots = (ns + EOM_NT - 1) / EOM_NT;
-    eom_post_fn fn = nullptr;
-    int tr = 1;
-    size_t lds = 0;
-    bool ok = false;
-    if (slots <= 4) ok = eom_post_try<4>(ns, &fn, &tr, &lds);
-    else if (slots <= 8) ok = eom_post_try<8>(ns, &fn, &tr, &lds);
-    else if (slots <= 16) ok = eom_post_try<16>(ns, &fn, &tr, &lds);
-    else if (slots <= 24) ok = eom_post_try<24>(ns, &fn, &tr, &lds);
-    else if (slots <= 32) ok = eom_post_try<32>(ns, &fn, &tr, &lds);
-    if (!ok) return KTM_ERR_UNSUPPORTED;
-    const int64_t nblk = (ntraces + tr - 1) / tr;
-    if (nblk > 0x7fffffffLL) return KTM_ERR_UNSUPPORTED;
+    if (ns > 8192) return KTM_ERR_UNSUPPORTED;
     EOM_TRY(hipSetDevice(p->device));
-    EOM_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds));
-    hipLaunchKernelGGL(fn, dim3((unsigned)nblk), dim3(EOM_NT), lds, (hipStream_t)stream, csp, fold,
-                       ntraces, ns, p->s.dt, p->d_g2, out);
-    EOM_TRY(hipGetLastError());
-    return KTM_OK;
+    const hipError_t e = postmx_launch<PostEom>(csp, fold, ntraces, ns, p->s.dt, p->d_g2, out,
+                                                (hipStream_t)stream);
+    return e == hipSuccess ? KTM_OK : e == hipErrorInvalidValue ? KTM_ERR_UNSUPPORTED : KTM_ERR_DEVICE;
 }
 
 extern "C" int ktm_eom_post(const ktm_eom_plan *p, const float *csp, const float *fold, float *out,
