@@ -11,10 +11,12 @@
  * to PM_WAVES * NTW * 16 samples (gridDim.y groups cover longer traces);
  * wave w owns NTW 16-row tiles.  x streams through LDS in chunks of PM_UC
  * samples; each v_mfma_f64_16x16x4f64 takes a 4 x 16 block of x (one LDS
- * read shared by the wave's NTW tiles) and a 16 x 4 block of H read from h,
- * ~0.5 B of LDS per FMA.  Operand layout of v_mfma_f64_16x16x4f64 for lane
- * l (measured with one-hot operands): A[l % 16][l / 16], B[l / 16][l % 16],
- * D[4 r + l / 16][l % 16] in result register r.
+ * read shared by the wave's NTW tiles) and a 16 x 4 block of H read from a
+ * window of h in LDS (the NTW reads of a k-step at one address plus
+ * immediate offsets), ~0.5 B of LDS per FMA.  Operand layout of
+ * v_mfma_f64_16x16x4f64 for lane l (measured with one-hot operands):
+ * A[l % 16][l / 16], B[l / 16][l % 16], D[4 r + l / 16][l % 16] in result
+ * register r.
  */
 #ifndef KTM_POSTMX_H
 #define KTM_POSTMX_H
@@ -28,25 +30,32 @@
 #define PM_LD 17                     /* padded row of the chunk (16 traces) */
 typedef double pm_d4 __attribute__((ext_vector_type(4)));
 
-template <class Pro, int NTW, bool HG>
+template <class Pro, int NTW>
 __global__ void __launch_bounds__(PM_WAVES * PM_WAVE)
 k_postmx(const float *__restrict__ in, const float *__restrict__ fold, int64_t ntraces, int ns,
          float dt, const double *__restrict__ h2, float *__restrict__ out)
 {
+    constexpr int RW = PM_WAVES * NTW * 16;            /* rows of a group */
     extern __shared__ double pm_lds[];
     double *xs = pm_lds;                               /* [PM_UC][PM_LD] */
-    double *hl = pm_lds + PM_UC * PM_LD;               /* [2 ns] when !HG */
-    const double *hh = HG ? h2 : hl;
+    double *hp = pm_lds + PM_UC * PM_LD;               /* [RW + ns + 3] window of h */
     const int tid = threadIdx.x, lane = tid & (PM_WAVE - 1), wv = tid / PM_WAVE;
     const int64_t tr0 = (int64_t)blockIdx.x * 16;
-    const int tw0 = (blockIdx.y * PM_WAVES + wv) * NTW * 16;   /* first row of the wave */
-    if (!HG)
-        for (int m = tid; m < 2 * ns; m += PM_WAVES * PM_WAVE) hl[m] = h2[m];
+    const int g0 = blockIdx.y * RW;                    /* first row of the group */
+    const int tw0 = g0 + wv * NTW * 16;                /* first row of the wave */
+    /* hp[k] = h[(k + g0 - 3) mod ns]: H[t][u] = h[(t - u) mod ns] =
+     * hp[t - g0 - u + ns + 3] for every row t of the group and every u < ns + 3,
+     * so the operand reads need no clamp */
+    for (int k = tid; k < RW + ns + 3; k += PM_WAVES * PM_WAVE) {
+        const int m = (k + g0 - 3) % ns;
+        hp[k] = h2[m < 0 ? m + ns : m];
+    }
     pm_d4 acc[NTW];
 #pragma unroll
     for (int i = 0; i < NTW; ++i) acc[i] = (pm_d4){0.0, 0.0, 0.0, 0.0};
     const int ar = lane & 15, ak = lane >> 4;          /* A row, k of this lane */
-    const int n2 = 2 * ns - 1;
+    /* A[ar][ak] of tile i at u = u0 + ak: hp[hbase - u0 + 16 i] */
+    const int hbase = (tw0 - g0) + ar - ak + ns + 3;
     for (int c0 = 0; c0 < ns; c0 += PM_UC) {
         __syncthreads();                               /* previous chunk consumed */
         /* x[u][j] of the chunk; 0 for u = 0, past the trace or past the image */
@@ -61,14 +70,10 @@ k_postmx(const float *__restrict__ in, const float *__restrict__ fold, int64_t n
         const int nu = min(PM_UC, ((ns - c0) + 3) & ~3);
         for (int k0 = 0; k0 < nu; k0 += 4) {
             const double b = xs[(k0 + ak) * PM_LD + ar];  /* B[ak][ar] = x[j=ar][c0+k0+ak] */
-            const int u = c0 + k0 + ak;
+            const double *ha = hp + (hbase - c0 - k0);
 #pragma unroll
-            for (int i = 0; i < NTW; ++i) {
-                /* A[ar][ak] = H[t][u] = h2[t - u + ns], clamped into h2 (rows
-                 * past ns are not stored; columns past ns meet x = 0) */
-                const int m = min(max(tw0 + i * 16 + ar - u + ns, 0), n2);
-                acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(hh[m], b, acc[i], 0, 0, 0);
-            }
+            for (int i = 0; i < NTW; ++i)
+                acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(ha[16 * i], b, acc[i], 0, 0, 0);
         }
     }
     /* D[4 r + l / 16][l % 16]: rows t, column j = trace */
@@ -96,14 +101,9 @@ static hipError_t postmx_launch(const float *in, const float *fold, int64_t ntra
     const int ntw = ns <= PM_WAVES * 4 * 16 ? 4 : ns <= PM_WAVES * 8 * 16 ? 8 : 12;
     const int rows = PM_WAVES * ntw * 16;
     const int groups = (ns + rows - 1) / rows;
-    /* h in LDS when it fits beside the x chunk, else read through L1/L2 */
-    const size_t lds_x = sizeof(double) * PM_UC * PM_LD;
-    const bool hg = lds_x + sizeof(double) * 2 * (size_t)ns > 150 * 1024;
-    const size_t lds = lds_x + (hg ? 0 : sizeof(double) * 2 * (size_t)ns);
-    fn_t fn;
-    if (ntw == 4) fn = hg ? k_postmx<Pro, 4, true> : k_postmx<Pro, 4, false>;
-    else if (ntw == 8) fn = hg ? k_postmx<Pro, 8, true> : k_postmx<Pro, 8, false>;
-    else fn = hg ? k_postmx<Pro, 12, true> : k_postmx<Pro, 12, false>;
+    const size_t lds = sizeof(double) * ((size_t)PM_UC * PM_LD + (size_t)rows + ns + 3);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    fn_t fn = ntw == 4 ? k_postmx<Pro, 4> : ntw == 8 ? k_postmx<Pro, 8> : k_postmx<Pro, 12>;
     const int64_t nblk = (ntraces + 15) / 16;
     if (nblk > 0x7fffffffLL || groups > 65535) return hipErrorInvalidValue;
     hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
