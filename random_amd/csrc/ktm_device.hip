@@ -17,7 +17,8 @@
  *                   in registers), streams the traces of a block through LDS in
  *                   trace order and accumulates F(floor(dsr/dt)) per sample
  *   k_postmx<PostKtm>  derivative/fold + low-pass, Kirchhoff.c:1457-1479, as
- *                   an FP64 matrix product on the matrix cores (ktm_postmx.h)
+ *                   an FP64 matrix product on the matrix cores (ktm_postmx.h);
+ *                   its transpose k_postmx<PostImg> + k_poststack_adjoint_tail
  *   k_image_*       image<job>.su trace records (headers + big-endian
  *                   samples), Kirchhoff.c:1437-1598
  *   k_demigrate     exact transpose of k_migrate (CLOSED prep)
@@ -1315,64 +1316,32 @@ k_demigrate(const DemigArgs a)
 
 /* Transpose of the post-stack chain image = P0 H C D km of Kirchhoff.c:
  * 1457-1479 (D: first difference with d[0] = 0, C: -dt^2 t / fold[t],
- * H: the even circular low-pass, P0: out[0] = 0):  km_bar = D^T C H P0 img. */
-template <int SLOTS, int TR, bool HG>
-__global__ void __launch_bounds__(PS_NT)
-k_poststack_adjoint(const float *__restrict__ img, const float *__restrict__ fold, int64_t ntraces,
-                    int ns, float dt, const double *__restrict__ h2, float *__restrict__ out)
+ * H: the even circular low-pass, P0: out[0] = 0):  km_bar = D^T C H P0 img.
+ * H is symmetric (h even), so w = H P0 img is the post-stack convolution of
+ * img itself (k_postmx<PostImg>, written into out); this kernel finishes a
+ * trace per workgroup in place: uu[t] = C[t] w[t] (FP64, uu[0] = uu[ns] = 0),
+ * out[t] = uu[t] - uu[t + 1]. */
+struct PostImg {
+    static __device__ __forceinline__ double x(const float *im, const float *, int u, float)
+    {
+        return (double)im[u];
+    }
+};
+
+__global__ void __launch_bounds__(PS_NT) k_poststack_adjoint_tail(const float *__restrict__ fold,
+                                                                  int ns, float dt, float *out)
 {
-    extern __shared__ double lds_d[];
-    double *zz = lds_d;                          /* [ns][TR] P0 img           */
-    double *uu = zz + (int64_t)ns * TR;          /* [ns+1][TR] C H P0 img     */
-    double *hl = uu + (int64_t)(ns + 1) * TR;    /* [2*ns] when !HG           */
-    const double *hh = HG ? h2 : hl;
-    const int64_t tr0 = (int64_t)blockIdx.x * TR;
-    if (!HG)
-        for (int m = threadIdx.x; m < 2 * ns; m += PS_NT) hl[m] = h2[m];
-    for (int e = threadIdx.x; e < ns * TR; e += PS_NT) {
-        const int u = e / TR, b = e - u * TR;
-        const int64_t tr = tr0 + b;
-        zz[u * TR + b] = tr < ntraces && u >= 1 ? (double)img[tr * ns + u] : 0.0;
+    extern __shared__ double uu[];                     /* [ns + 1] */
+    const int64_t tr = blockIdx.x;
+    float *o = out + tr * ns;
+    const float *f = fold + tr * ns;
+    for (int t = threadIdx.x; t <= ns; t += PS_NT) {
+        double v = 0.0;
+        if (t >= 1 && t < ns) v = -(double)dt * (double)dt * (double)t / (double)f[t] * (double)o[t];
+        uu[t] = v;
     }
     __syncthreads();
-    double acc[SLOTS][TR];
-#pragma unroll
-    for (int s = 0; s < SLOTS; ++s)
-#pragma unroll
-        for (int b = 0; b < TR; ++b) acc[s][b] = 0.0;
-    for (int u = 0; u < ns; ++u) {
-        double zv[TR];
-#pragma unroll
-        for (int b = 0; b < TR; ++b) zv[b] = zz[u * TR + b];
-#pragma unroll
-        for (int s = 0; s < SLOTS; ++s) {
-            const int t = s * PS_NT + threadIdx.x;
-            /* H is symmetric: (H^T z)[t] = sum_u h[(u - t) mod ns] z[u] = sum_u h[(t - u) mod ns] z[u] */
-            const double hv = t < ns ? hh[t - u + ns] : 0.0;
-#pragma unroll
-            for (int b = 0; b < TR; ++b) acc[s][b] = __builtin_fma(hv, zv[b], acc[s][b]);
-        }
-    }
-#pragma unroll
-    for (int s = 0; s < SLOTS; ++s) {
-        const int t = s * PS_NT + threadIdx.x;
-        if (t < ns)
-#pragma unroll
-            for (int b = 0; b < TR; ++b) {
-                const int64_t tr = tr0 + b;
-                double c = 0.0;
-                if (tr < ntraces && t >= 1)
-                    c = -(double)dt * (double)dt * (double)t / (double)fold[tr * ns + t];
-                uu[t * TR + b] = c * acc[s][b];
-            }
-    }
-    for (int b = threadIdx.x; b < TR; b += PS_NT) uu[ns * TR + b] = 0.0;
-    __syncthreads();
-    for (int e = threadIdx.x; e < ns * TR; e += PS_NT) {
-        const int t = e / TR, b = e - t * TR;
-        const int64_t tr = tr0 + b;
-        if (tr < ntraces) out[tr * ns + t] = (float)(uu[t * TR + b] - uu[(t + 1) * TR + b]);
-    }
+    for (int t = threadIdx.x; t < ns; t += PS_NT) o[t] = (float)(uu[t] - uu[t + 1]);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1958,61 +1927,6 @@ extern "C" int ktm_migrate_block(const ktm_plan *p, const void *prepared, int dt
     return KTM_OK;
 }
 
-/* post-stack kernel choice: traces per workgroup and where h lives, so the
- * LDS (160 KB) holds the work arrays; SLOTS * 256 >= ns */
-typedef void (*ps_fn)(const float *, const float *, int64_t, int, float, const double *, float *);
-struct PsChoice {
-    ps_fn fn;
-    int tr;
-    size_t lds;
-};
-
-template <int SLOTS, int TR, bool HG>
-struct PsAdj {
-    static constexpr ps_fn fn = k_poststack_adjoint<SLOTS, TR, HG>;
-};
-
-template <template <int, int, bool> class Kern, int SLOTS>
-static bool ps_try(int ns, bool adjoint, PsChoice *c)
-{
-    /* doubles per traces-per-workgroup TR: ns TR (+ 2 ns for h) for a
-     * convolution alone, (2 ns + 1) TR (+ 2 ns) for the transpose */
-    const size_t cap = 160 * 1024 / sizeof(double);
-    const size_t per = adjoint ? (size_t)(2 * ns + 1) : (size_t)ns, h = 2 * (size_t)ns;
-    if (4 * per + h <= cap) { *c = {Kern<SLOTS, 4, false>::fn, 4, sizeof(double) * (4 * per + h)}; return true; }
-    if (per + h <= cap) { *c = {Kern<SLOTS, 1, false>::fn, 1, sizeof(double) * (per + h)}; return true; }
-    if (per <= cap) { *c = {Kern<SLOTS, 1, true>::fn, 1, sizeof(double) * per}; return true; }
-    return false;
-}
-
-template <template <int, int, bool> class Kern>
-static int ps_pick(int ns, bool adjoint, PsChoice *c)
-{
-    const int slots = (ns + PS_NT - 1) / PS_NT;
-    bool ok = false;
-    if (slots <= 4) ok = ps_try<Kern, 4>(ns, adjoint, c);
-    else if (slots <= 8) ok = ps_try<Kern, 8>(ns, adjoint, c);
-    else if (slots <= 12) ok = ps_try<Kern, 12>(ns, adjoint, c);
-    else if (slots <= 16) ok = ps_try<Kern, 16>(ns, adjoint, c);
-    else if (slots <= 24) ok = ps_try<Kern, 24>(ns, adjoint, c);
-    return ok ? KTM_OK : KTM_ERR_UNSUPPORTED;
-}
-
-static int ps_launch(const ktm_plan *p, const PsChoice &c, const float *in, const float *fold,
-                     float *out, void *stream)
-{
-    const ktm_setup *s = &p->tb.s;
-    const int64_t ntraces = (int64_t)s->nxp * s->nyp * s->nh;
-    const int64_t nblk = (ntraces + c.tr - 1) / c.tr;
-    if (nblk > 0x7fffffffLL) return KTM_ERR_UNSUPPORTED;
-    HIP_TRY(hipFuncSetAttribute((const void *)c.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)c.lds));
-    hipLaunchKernelGGL(c.fn, dim3((unsigned)nblk), dim3(PS_NT), c.lds, (hipStream_t)stream, in,
-                       fold, ntraces, s->ns, s->dt, p->d_hconv2, out);
-    HIP_TRY(hipGetLastError());
-    return KTM_OK;
-}
-
 extern "C" int ktm_poststack(const ktm_plan *p, const float *km, const float *fold, float *out,
                              void *stream)
 {
@@ -2180,7 +2094,15 @@ extern "C" int ktm_poststack_adjoint(const ktm_plan *p, const float *img, const 
                                      float *km_bar, void *stream)
 {
     if (!p || !img || !fold || !km_bar) return KTM_ERR_ARG;
-    PsChoice c;
-    const int st = ps_pick<PsAdj>(p->tb.s.ns, true, &c);
-    return st != KTM_OK ? st : ps_launch(p, c, img, fold, km_bar, stream);
+    const ktm_setup *s = &p->tb.s;
+    const int64_t ntraces = (int64_t)s->nxp * s->nyp * s->nh;
+    if (ntraces > 0x7fffffffLL) return KTM_ERR_UNSUPPORTED;
+    hipStream_t hs = (hipStream_t)stream;
+    const hipError_t e = postmx_launch<PostImg>(img, fold, ntraces, s->ns, s->dt, p->d_hconv2,
+                                                km_bar, hs);
+    if (e != hipSuccess) return e == hipErrorInvalidValue ? KTM_ERR_UNSUPPORTED : KTM_ERR_DEVICE;
+    hipLaunchKernelGGL(k_poststack_adjoint_tail, dim3((unsigned)ntraces), dim3(PS_NT),
+                       sizeof(double) * (size_t)(s->ns + 1), hs, fold, s->ns, s->dt, km_bar);
+    HIP_TRY(hipGetLastError());
+    return KTM_OK;
 }
