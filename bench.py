@@ -402,7 +402,7 @@ def poststack_leg(dev, plan=None, reps=3):
 
 
 def eom_leg(dev, reps=3, cpu_traces=48):
-    """Equivalent-offset migration (EOM.c csp()) of the configuration-0 line:
+    """Equivalent-offset migration (EOM.c csp()) of the 40-shot line:
     4800 traces x 1001 samples into 800 CSPs x 160 offset bins, plus the post
     filter; the CPU oracle timed on the first traces of the same survey."""
     import numpy as np
@@ -438,7 +438,7 @@ def eom_leg(dev, reps=3, cpu_traces=48):
                                 cfg.velocity(sv.ns))
     el = time.perf_counter() - t0
     log(f"eom: binning {ms_b:.2f} ms, post {ms_p:.2f} ms, {n:.3e} samples; cpu {ncpu:.3e} in {el:.2f}s")
-    return {"workload": "configuration-0 line, 4800 traces x 1001 samples -> 800 CSPs x 160 "
+    return {"workload": "40-shot line, 4800 traces x 1001 samples -> 800 CSPs x 160 "
                         "offset bins of 12.5 m, aperture 2000 m, v = 2000 m/s",
             "evaluated_samples": n, "ms_binning": round(ms_b, 2), "ms_post": round(ms_p, 2),
             "G_samples_per_s": round(n / (ms_b * 1e-3) / 1e9, 2),

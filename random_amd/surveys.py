@@ -64,7 +64,7 @@ def _velocity_grid(cfg: Config, ns: int, dt: float, v0: float, vt: float, vx: fl
 def line2d(nshots=40, nrecv_side=60, shot_dx=50.0, recv_dx=12.5, ns=1001, ncols=800,
            col_dx=6.25, aperture=2000.0, nscat=20, seed=20240001, jobs_x=1,
            nh=1, oh=0.0, dh=2000.0, x0=1522.0, t0_range=(0.3, 3.5)) -> Survey:
-    """2-D split-spread line (configuration 0 shape by default): shots every
+    """2-D split-spread line (by default 40 shots x 120 receivers, 1001 samples): shots every
     ``shot_dx`` m, ``nrecv_side`` receivers each side at ``recv_dx`` m,
     ``ncols`` image columns at ``col_dx`` m starting at 0, v = 1500 + 400 t.
     Coordinates are stored in centimetres with scale_xy = 0.01
@@ -95,7 +95,7 @@ def line2d(nshots=40, nrecv_side=60, shot_dx=50.0, recv_dx=12.5, ns=1001, ncols=
 
 def area3d(ntr=1_000_000, ns=1500, ncols=400, col_dx=25.0, extent=10_000.0, max_offset=4000.0,
            aperture=2500.0, nscat=200, seed=20240011, jobs_x=1, jobs_y=1) -> Survey:
-    """3-D survey (configuration 1 shape by default): shots uniform over an
+    """3-D survey (by default 1e6 traces, 1500 samples, 400 x 400 image): shots uniform over an
     ``extent`` square, offsets U(0, max_offset), azimuths U(0, 2 pi); integer
     metre coordinates shifted by +5000 m with ox = oy = 5000 (header words stay
     positive); image ncols x ncols at col_dx from 0; v = 1500 + 350 t + 0.02 IX."""
@@ -158,13 +158,13 @@ def traces_numpy(sv: Survey, noise: bool = True, chunk: int = 512) -> np.ndarray
 
 
 def line2d_lsm(seed=20240021) -> Survey:
-    """Configuration 2: the 2-D line at 200 shots x 240 receivers, 1501
+    """The adjoint + LSM line: 200 shots x 240 receivers, 1501
     samples, 1600 image columns; 50 point scatterers for the reflectivity."""
     return line2d(nshots=200, nrecv_side=120, ns=1501, ncols=1600, nscat=50, seed=seed)
 
 
 def line2d_vgrad(seed=20240041, **kw) -> Survey:
-    """Configuration 4: 600 shots x 240 split-spread receivers (144,000
+    """The velocity-gradient line: 600 shots x 240 split-spread receivers (144,000
     traces), 2001 samples, 2000 image columns at 16 m, 64 offset classes of
     23.5 m (class = floor((offset - 6.25) / 23.5), offsets 12.5..1500 m),
     aperture 2000 m, 50 scatterers."""
@@ -175,7 +175,7 @@ def line2d_vgrad(seed=20240041, **kw) -> Survey:
 
 
 def line2d_eom(seed=20240001):
-    """The configuration-0 line (40 shots x 120 receivers, 1001 samples) with
+    """The 40-shot line (40 shots x 120 receivers, 1001 samples) with
     an equivalent-offset migration set-up: 800 CSPs at 6.25 m from 0 (the KTM
     image columns), 160 offset bins of 12.5 m, aperture 2000 m, and the
     constant velocity of EOM.c's own model (2000 m/s).  Returns (survey,

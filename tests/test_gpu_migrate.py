@@ -27,22 +27,22 @@ def say(*a):
 
 
 @pytest.fixture(scope="module")
-def line0():
-    """Configuration 0: 40 shots x 120 receivers, ns=1001, 800 columns, 4 job slabs."""
+def line40():
+    """The 40-shot line: 40 shots x 120 receivers, ns=1001, 800 columns, 4 job slabs."""
     sv = surveys.line2d(jobs_x=4)
     tr = surveys.traces_numpy(sv)
     return sv, tr
 
 
 @pytest.fixture(scope="module")
-def line0_oracle(line0):
-    sv, tr = line0
+def line40_oracle(line40):
+    sv, tr = line40
     km, fold, n = oracle.migrate(sv.cfg, sv.ns, sv.dt, tr, sv.coords, sv.vel)
     return km, fold, n
 
 
-def test_prep_reference_bitexact(cuda, line0):
-    sv, tr = line0
+def test_prep_reference_bitexact(cuda, line40):
+    sv, tr = line40
     say("case: prep REFERENCE, line2d, 300 traces")
     plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
     sel = np.arange(0, sv.ntr, 16)[:300]
@@ -61,10 +61,10 @@ def test_prep_reference_bitexact(cuda, line0):
         assert not prepared[k, sv.ns:].any()
 
 
-def test_migrate_line0_bitexact(cuda, line0, line0_oracle):
-    sv, tr = line0
-    km_o, fold_o, n_o = line0_oracle
-    say("case: migrate line2d configuration 0, 4800 traces, ns=1001, 4 jobs")
+def test_migrate_line40_bitexact(cuda, line40, line40_oracle):
+    sv, tr = line40
+    km_o, fold_o, n_o = line40_oracle
+    say("case: migrate the 40-shot line, 4800 traces, ns=1001, 4 jobs")
     plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
     km, fold, n = plan.migrate(torch.from_numpy(tr).to(cuda), torch.from_numpy(sv.coords).to(cuda))
     km = km.cpu().numpy()
@@ -78,9 +78,9 @@ def test_migrate_line0_bitexact(cuda, line0, line0_oracle):
     assert same == 1.0
 
 
-def test_blocks_and_jobs_equal_single_pass(cuda, line0, line0_oracle):
-    sv, tr = line0
-    km_o, fold_o, _ = line0_oracle
+def test_blocks_and_jobs_equal_single_pass(cuda, line40, line40_oracle):
+    sv, tr = line40
+    km_o, fold_o, _ = line40_oracle
     say("case: line2d in blocks of 700 traces, and per-job regions")
     plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
     d_tr = torch.from_numpy(tr).to(cuda)
@@ -115,10 +115,10 @@ def test_migrate_area3d_small_bitexact(cuda):
     assert same == 1.0
 
 
-def test_poststack_matches_oracle(cuda, line0, line0_oracle):
-    sv, _ = line0
-    km_o, fold_o, _ = line0_oracle
-    say("case: post-stack filter on the configuration-0 image")
+def test_poststack_matches_oracle(cuda, line40, line40_oracle):
+    sv, _ = line40
+    km_o, fold_o, _ = line40_oracle
+    say("case: post-stack filter on the 40-shot line's image")
     plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
     out = plan.poststack(torch.from_numpy(km_o).to(cuda), torch.from_numpy(fold_o).to(cuda))
     tm, aa = oracle.alias_filter(sv.cfg, sv.ns, sv.dt)
