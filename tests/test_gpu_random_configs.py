@@ -44,7 +44,7 @@ def random_case(seed):
     return sv, cfg, block
 
 
-@pytest.mark.parametrize("seed", range(12))
+@pytest.mark.parametrize("seed", range(48))
 def test_random_configuration_bitexact(cuda, seed):
     sv, cfg, block = random_case(1000 + seed)
     print(f"case: seed {1000 + seed}: ns {sv.ns}, {sv.ntr} traces, {cfg.end_iline - cfg.start_iline}"
