@@ -67,3 +67,30 @@ def test_random_configuration_bitexact(cuda, seed):
     ref = oracle.poststack(km_o, fold_o, sv.dt, aa)
     e = float(np.linalg.norm(img - ref) / max(np.linalg.norm(ref), 1e-30))
     assert e <= 1e-6
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_configuration_dot_product(cuda, seed):
+    """The adjoint pair on the same random configurations: <L^T d, m> = <d, L m>
+    for seeded Gaussian d and m.  The error is measured against the
+    Cauchy-Schwarz scale |L^T d| |m| (to 1e-8), and relative to the product
+    itself (to 1e-5) unless the product is near zero by chance: on random
+    configurations a product below 1e-4 of that scale occurs (3 of 500 seeds,
+    with the scaled error still ~1e-10)."""
+    from random_amd.lsm import Operator, dot64
+    sv, cfg, _ = random_case(1000 + seed)
+    plan = Plan(cfg, sv.ns, sv.dt, sv.vel)
+    op = Operator(plan, torch.from_numpy(sv.coords).to(cuda))
+    g = torch.Generator(device=cuda).manual_seed(seed)
+    d = torch.randn((sv.ntr, sv.ns), generator=g, device=cuda)
+    m = torch.randn(plan.image_shape, generator=g, device=cuda)
+    a = op.adjoint(d)
+    lhs, rhs = dot64(a, m), dot64(d, op.forward(m))
+    scale = float(a.double().norm() * m.double().norm())
+    err = abs(lhs - rhs)
+    print(f"case: dot test, seed {1000 + seed}: rel {err / abs(lhs):.2e}, scaled {err / scale:.2e}, "
+          f"|lhs| / scale {abs(lhs) / scale:.2e}", flush=True)
+    assert scale > 0.0
+    assert err <= 1e-8 * scale
+    if abs(lhs) >= 1e-4 * scale:
+        assert err <= 1e-5 * abs(lhs)
