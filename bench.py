@@ -401,6 +401,29 @@ def poststack_leg(dev, plan=None, reps=3):
             "image_records_hbm_frac": round(bytes_r / (ms_r * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)}
 
 
+def host_leg(sv, traces, ntr=100_000):
+    """The host-array path (ktm_migrate, the one-call driver): the first ntr
+    traces of the 3-D survey from pageable host memory, plan set-up (the
+    0.96 GB velocity model uploaded), blocks of ntblock traces copied in and
+    migrated into the whole image, the post-stack filter, and km, fold and
+    image copied back -- the PCIe-inclusive rate beside the kernels' own."""
+    from random_amd.ktm import migrate_host
+    tr = traces[:ntr].cpu().numpy()
+    co = sv.coords[:ntr]
+    km, fold, image, st = migrate_host(sv.cfg, sv.ns, sv.dt, tr, co, sv.vel)
+    n = int(st["contributions"])
+    img_bytes = km.nbytes + fold.nbytes + image.nbytes
+    del km, fold, image
+    log(f"host path: {ntr} traces, {st['total_ms']:.0f} ms total, {st['migrate_ms']:.0f} ms in "
+        f"the migration kernels")
+    return {"traces": ntr, "contributions": n, "ms_total": round(st["total_ms"], 1),
+            "ms_migration_kernels": round(st["migrate_ms"], 1),
+            "G_contributions_per_s_total": round(n / (st["total_ms"] * 1e-3) / 1e9, 1),
+            "G_contributions_per_s_kernels": round(n / (st["migrate_ms"] * 1e-3) / 1e9, 1),
+            "host_bytes_in": int(tr.nbytes + co.nbytes + sv.vel.nbytes),
+            "host_bytes_out": img_bytes}
+
+
 def eom_leg(dev, reps=3, cpu_traces=48):
     """Equivalent-offset migration (EOM.c csp()) of the 40-shot line:
     4800 traces x 1001 samples into 800 CSPs x 160 offset bins, plus the post
@@ -582,6 +605,7 @@ def main():
         result["mixed_precision"] = legs
     if not args.profile and not args.no_extra:
         result["ingest"] = ingest_leg(sv, plan, traces, args.block, dev)
+        result["host_path"] = host_leg(sv, traces)
         result["velocity_gradient"] = vgrad_leg(dev)
         result["adjoint_lsm"] = adjoint_leg(dev)
         result["poststack"] = poststack_leg(dev, plan)
