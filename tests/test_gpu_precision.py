@@ -91,8 +91,9 @@ def _bf16_rne(v32: np.ndarray) -> np.ndarray:
 @pytest.mark.parametrize("dtype", [TRACE_BF16, TRACE_F16], ids=["bf16", "fp16"])
 def test_quantized_storage_matches_numpy_error_feedback(cuda, dtype):
     """k_quantize against a numpy restatement: scale = max|tr|, v = tr/scale
-    (times the float reciprocal) + carried error, RNE to 16 bits, error =
-    v - decoded; 130 traces (a partial workgroup) of 777 samples."""
+    (times the float reciprocal) + the sixth-order feedback of the previous
+    rounding errors, RNE to 16 bits, error = v - decoded; 130 traces (a
+    partial workgroup) of 777 samples."""
     rng = np.random.default_rng(21)
     ntr, ns = 130, 777
     tr = (rng.standard_normal((ntr, ns)) * rng.uniform(1e-3, 1e3, (ntr, 1))).astype(np.float32)
@@ -108,10 +109,13 @@ def test_quantized_storage_matches_numpy_error_feedback(cuda, dtype):
     scale = np.where(m > 0, m, np.float32(1.0)).astype(np.float32)
     assert np.array_equal(scale_gpu.view(np.uint32), scale.view(np.uint32))
     inv = (np.float32(1.0) / scale).astype(np.float32)
-    e = np.zeros(ntr, np.float32)
+    qa = [np.float32(c) for c in (1.625, -1.625, 1.5, -1.0, 0.625, -0.125)]
+    e = [np.zeros(ntr, np.float32) for _ in range(6)]              # e[t-1] .. e[t-6]
     want = np.zeros((ntr, plan.row), np.uint16)
     for t in range(ns):
-        v = (tr[:, t] * inv + e).astype(np.float32)
+        fb = ((((qa[0] * e[0] + qa[1] * e[1]) + qa[2] * e[2]) + qa[3] * e[3]) + qa[4] * e[4]) \
+            + qa[5] * e[5]
+        v = (tr[:, t] * inv + fb).astype(np.float32)
         if dtype == TRACE_BF16:
             h = _bf16_rne(v)
             dec = (h.astype(np.uint32) << 16).view(np.float32)
@@ -120,7 +124,7 @@ def test_quantized_storage_matches_numpy_error_feedback(cuda, dtype):
             h = h16.view(np.uint16)
             dec = h16.astype(np.float32)
         want[:, t] = h
-        e = (v - dec).astype(np.float32)
+        e = [(v - dec).astype(np.float32)] + e[:5]
     assert np.array_equal(got, want)
 
 
