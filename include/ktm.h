@@ -134,7 +134,8 @@ int64_t ktm_plan_image_size(const ktm_plan *plan);
  * and, for F32 storage, prepared[ntr][trace_row] floats holding F(T) for
  * T < ns (zero beyond).  For BF16/F16 storage (CLOSED prep only), prepared
  * [ntr][trace_row] holds the raw trace divided by its max-abs (geom.scale),
- * rounded with first-order error feedback; ktm_migrate_block decodes and
+ * rounded with sixth-order error feedback that shapes the rounding noise
+ * against the [1,2,1] taps; ktm_migrate_block decodes and
  * filters such a block into an FP32 workspace owned by the plan before
  * migrating it.  Replaces Kirchhoff.c:1296-1328. */
 int ktm_prep(const ktm_plan *plan, const float *traces, const int32_t *coords,
