@@ -596,6 +596,10 @@ __device__ __forceinline__ bool fast_ambiguous(float qc, float eps = KTM_FAST_EP
 /* floor of the clamped q, in [0, ns] */
 __device__ __forceinline__ int fast_T(float qc) { return (int)qc; }
 
+/* staged batches whose taps the Kahan variants sum in plain FP32 before one
+ * compensated step */
+#define KTM_KAHAN_FOLD 4
+
 /* One workgroup = 8 waves = NCOL image columns x W time segments.  Each lane
  * owns K samples t = seg*64K + 64k + lane of its column: the image values
  * (km, fold count, Kahan compensation), t2[t] and 1/v^2 stay in registers
@@ -692,6 +696,26 @@ k_migrate(const MigArgs a)
         }
     }
 
+    /* Kahan variants: the taps are summed onto y = -c in plain FP32 over
+     * KTM_KAHAN_FOLD staged batches (at most KTM_KAHAN_FOLD * B terms, error
+     * <= (terms - 1) u times their magnitudes) and added to the image with one
+     * compensated step, which keeps the compensated error bound of order
+     * u * sum |f| at a fraction of the per-tap cost.  Plain variants add
+     * every tap into the image in trace order, as the reference does. */
+    float y[KAHAN ? K : 1];
+#pragma unroll
+    for (int k = 0; k < (KAHAN ? K : 1); ++k) y[k] = -cmp[k];
+    int nfold = 0;
+    auto fold_y = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int k = 0; k < (KAHAN ? K : 1); ++k) {
+            const float yk = y[k];
+            const float s = acc[k] + yk;
+            y[k] = yk - (s - acc[k]);           /* -c, c = (s - acc) - y */
+            acc[k] = s;
+        }
+    };
+
     uint32_t npass = 0;
     const int n4 = ldl / 4;
     for (int64_t c0 = 0; c0 < a.ntr; c0 += NT) {
@@ -779,16 +803,6 @@ k_migrate(const MigArgs a)
                     }
                     return Fr[T];
                 };
-                /* Kahan variants: the taps of the batch's passing traces are
-                 * summed onto y = -c in plain FP32 (at most B terms, error
-                 * <= (B-1)u times their magnitudes) and added to the image
-                 * with one compensated step per batch, which keeps the
-                 * compensated error bound of order u * sum |f| at a fraction
-                 * of the per-tap cost.  Plain variants add every tap into
-                 * the image in trace order, as the reference does. */
-                float y[KAHAN ? K : 1];
-#pragma unroll
-                for (int k = 0; k < (KAHAN ? K : 1); ++k) y[k] = -cmp[k];
                 while (pm) {
                     const int j = __builtin_ctzll(pm);   /* traces in order */
                     pm &= pm - 1ull;
@@ -802,19 +816,15 @@ k_migrate(const MigArgs a)
                         else acc[k] = acc[k] + f;
                     }
                 }
-                if (KAHAN) {
-#pragma unroll
-                    for (int k = 0; k < K; ++k) {
-                        const float yk = y[KAHAN ? k : 0];
-                        const float s = acc[k] + yk;
-                        cmp[k] = (s - acc[k]) - yk;
-                        acc[k] = s;
-                    }
+                if (KAHAN && ++nfold == KTM_KAHAN_FOLD) {
+                    nfold = 0;
+                    fold_y();
                 }
             }
             __syncthreads();
         }
     }
+    if (KAHAN && nfold) fold_y();
 
     if (col_ok) {
 #pragma unroll
@@ -825,7 +835,7 @@ k_migrate(const MigArgs a)
                 /* fold adds the passing traces whose sample was inside the
                  * trace (Kirchhoff.c:1387-1396): all of them minus cnt */
                 a.fold[pix + t] = a.fold[pix + t] + (float)((int)npass - cnt[k]);
-                if (KAHAN) a.comp[pix + t] = cmp[k];
+                if (KAHAN) a.comp[pix + t] = -y[KAHAN ? k : 0];
             }
         }
         if (lane == 0 && seg == 0 && a.counter)
