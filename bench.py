@@ -111,6 +111,10 @@ def measured_peaks(dev, reps=5):
     return copy_gbs, lds_gbs
 
 
+# ---- CPU-baseline leg: the only code in this file that runs the oracle
+# (oracle/), as the CPU timing beside the GPU figures and as the checker of
+# the GPU precision modes; every GPU figure comes from libktm.so. ----
+
 def cpu_baseline(sv, traces_np, target_s, nthreads):
     """Oracle FP32 (the reference algorithm, strict IEEE, -O3) on the first S
     traces over the whole image, jobs = one slab per thread as the reference's
@@ -147,6 +151,33 @@ def cpu_baseline(sv, traces_np, target_s, nthreads):
                                       f"-ffast-math (Kirchhoff.c:3), {el4:.1f} s"}
 
 
+def cpu_baseline_fp64(sv, cfg, tr_np, co_np):
+    """CPU-baseline leg, checker part: the FP64 oracle image (FP32
+    traveltime index, FP64 data path) of a sub-volume, against which the
+    GPU precision modes are measured.  Returns (km64, contributions)."""
+    import oracle
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    km64, _, n64 = oracle.migrate(cfg, sv.ns, sv.dt, tr_np, co_np, sv.vel, mode=oracle.FP64_DATA,
+                                  nthreads=nthreads)
+    log(f"fp64 oracle on the sub-volume: {n64:.3e} contributions in "
+        f"{time.perf_counter() - t0:.1f}s")
+    return km64, n64
+
+
+def cpu_baseline_eom(cfg, sv, tr, coords):
+    """CPU-baseline leg for equivalent-offset migration: the oracle's csp()
+    over the given traces, one thread.  Returns (samples, seconds)."""
+    import numpy as np
+
+    import oracle
+    t0 = time.perf_counter()
+    _, _, ncpu = oracle.eom_all(cfg, sv.ns, np.float32(sv.dt), tr, coords, cfg.velocity(sv.ns))
+    return ncpu, time.perf_counter() - t0
+
+# ---- end of the CPU-baseline leg ----
+
+
 def precision_errors(sv, plan, traces, coords, ntr, dev):
     """Errors of every path against the FP64 oracle (FP32 traveltime index,
     FP64 data path) on the 50 x 50 x 1500 sub-volume at columns 175..224 of
@@ -154,21 +185,13 @@ def precision_errors(sv, plan, traces, coords, ntr, dev):
     import numpy as np
     import torch
 
-    import oracle
     from random_amd import (ACCUM_KAHAN, ACCUM_PLAIN, PREP_CLOSED, PREP_REFERENCE, TRACE_BF16,
                             TRACE_F16, TRACE_F32)
     from random_amd.ktm import Plan
     cfg = sv.cfg.replace(start_iline=175, end_iline=225, start_xline=175, end_xline=225,
                          jobs_x=10, jobs_y=2)
     sub = Plan(cfg, sv.ns, sv.dt, sv.vel)
-    tr_np = traces[:ntr].cpu().numpy()
-    co_np = sv.coords[:ntr]
-    nthreads = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
-    t0 = time.perf_counter()
-    km64, _, n64 = oracle.migrate(cfg, sv.ns, sv.dt, tr_np, co_np, sv.vel, mode=oracle.FP64_DATA,
-                                  nthreads=nthreads)
-    log(f"fp64 oracle on the sub-volume: {n64:.3e} contributions in "
-        f"{time.perf_counter() - t0:.1f}s")
+    km64, n64 = cpu_baseline_fp64(sv, cfg, traces[:ntr].cpu().numpy(), sv.coords[:ntr])
     out = {"error_check": f"50x50x{sv.ns} sub-volume (columns 175-224 x 175-224), first {ntr} "
                           f"traces, {n64:.3e} contributions; reference = CPU oracle with FP64 data "
                           f"path and the reference FP32 traveltime index"}
@@ -431,7 +454,6 @@ def eom_leg(dev, reps=3, cpu_traces=48):
     import numpy as np
     import torch
 
-    import oracle
     from random_amd import surveys
     from random_amd.eom import EomPlan
     sv, cfg = surveys.line2d_eom()
@@ -455,11 +477,7 @@ def eom_leg(dev, reps=3, cpu_traces=48):
         ms_p.append(e1.elapsed_time(e2))
     n = int(counter.item())
     ms_b, ms_p = float(np.median(ms_b)), float(np.median(ms_p))
-    sub = slice(0, cpu_traces)
-    t0 = time.perf_counter()
-    _, _, ncpu = oracle.eom_all(cfg, sv.ns, np.float32(sv.dt), tr[sub], sv.coords[sub],
-                                cfg.velocity(sv.ns))
-    el = time.perf_counter() - t0
+    ncpu, el = cpu_baseline_eom(cfg, sv, tr[:cpu_traces], sv.coords[:cpu_traces])
     log(f"eom: binning {ms_b:.2f} ms, post {ms_p:.2f} ms, {n:.3e} samples; cpu {ncpu:.3e} in {el:.2f}s")
     return {"workload": "40-shot line, 4800 traces x 1001 samples -> 800 CSPs x 160 "
                         "offset bins of 12.5 m, aperture 2000 m, v = 2000 m/s",

@@ -105,7 +105,9 @@ enum { KTM_PREP_REFERENCE = 0, KTM_PREP_CLOSED = 1 };
 /* Storage type of the traces the migration kernel stages. */
 enum { KTM_TRACE_F32 = 0, KTM_TRACE_BF16 = 1, KTM_TRACE_F16 = 2 };
 /* Image accumulation: plain FP32 adds in trace order (Kirchhoff.c:1394),
- * or FP32 with a Kahan compensation image. */
+ * or FP32 with a Kahan compensation image (the taps of up to 32 staged
+ * traces are summed in FP32, then added with one compensated step, so the
+ * error stays of order 2^-24 sum |tap|). */
 enum { KTM_ACCUM_PLAIN = 0, KTM_ACCUM_KAHAN = 1 };
 
 /* Device-resident setup: velocity model (squared, Kirchhoff.c:617), traveltime
@@ -178,7 +180,8 @@ int ktm_image_reset(const ktm_plan *plan, float *km, float *fold, float *comp,
  * the trace loop of Kirchhoff.c:1292-1406 for every job slab at once.
  * km/fold/comp are [nxp][nyp][nh][ns]; comp is needed for KAHAN only.
  * Accumulation continues from the current contents, in trace order, so
- * successive blocks equal one pass over their concatenation.  If counter is
+ * successive blocks equal one pass over their concatenation (bit for bit
+ * with PLAIN; with KAHAN within the compensated bound).  If counter is
  * non-NULL (device uint64) the number of contributions is added to it: ns per
  * (trace, column) pair through the aperture gates of Kirchhoff.c:1338,1354,
  * traces outside the offset classes included (the reference evaluates their

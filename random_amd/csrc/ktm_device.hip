@@ -1845,7 +1845,7 @@ static mig_fn pick_migrate(int ns, int accum, int64_t wgs8, int *W, int *K, int 
         if (2 * wgs8 < 512) { *W = 4; *K = 4; return pick_accum<4, 4>(accum, B); }
         *W = 2; *K = 8; return pick_accum<8, 2>(accum, B);
     }
-    /* (the Kahan K = 12 variant runs at 0.96x the plain one; four waves of
+    /* (the Kahan K = 12 variant runs at 0.97x the plain one; four waves of
      * K = 6 per column measured 0.72x, of K = 8 0.63x, before the Kahan sums
      * were blocked per batch) */
     if (ns <= 1536) { *W = 2; *K = 12; return pick_accum<12, 2>(accum, B); }
