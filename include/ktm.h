@@ -346,17 +346,44 @@ int ktm_eom_job_records(const ktm_eom_plan *plan, const float *post, int job, vo
                         int64_t *nrec, void *stream);
 
 /* Host convenience: whole reference job loop for all jobs (Kirchhoff.c:
- * 347-350 on one rank): prep + migrate in blocks of cfg->ntblock + post-stack.
- * Host arrays; km/fold/image are [nxp][nyp][nh][ns] (any may be NULL). */
+ * 347-350 on one rank): prep + migrate in blocks of cfg->ntblock + post-stack,
+ * on host arrays.  Replaces main()'s setup plus its kirchhoff(job) loop
+ * (Kirchhoff.c:254-350) with explicit arguments and a status code. */
 typedef struct ktm_stats {
     uint64_t contributions;   /* (trace, column) pairs in aperture x ns */
     double   migrate_ms;      /* device time of the migration kernels   */
     double   total_ms;        /* host wall time of the call             */
 } ktm_stats;
 
-int ktm_migrate(const ktm_config *cfg, int ns, float dt, const float *traces,
-                const int32_t *coords, int64_t ntr, const float *vel, int nsm,
-                float dtm, float *km, float *fold, float *image, ktm_stats *stats);
+/* The input traces as the reference reads them (Kirchhoff.c:1276-1310):
+ * ntr traces of ns FP32 samples at dt seconds and the header words sx, sy,
+ * gx, gy of each (bytes 73-88 of the trace header, as integers). */
+typedef struct ktm_traces {
+    const float   *samples;   /* [ntr][ns] */
+    const int32_t *coords;    /* [ntr][4]  */
+    int64_t        ntr;
+    int32_t        ns;
+    float          dt;        /* s */
+} ktm_traces;
+
+/* The RMS velocity cube of build_vel_model (Kirchhoff.c:449-626), in m/s
+ * (not squared), [inline][crossline][nsm] at dtm seconds per sample. */
+typedef struct ktm_velocity {
+    const float *v;
+    int32_t      nsm;
+    float        dtm;
+} ktm_velocity;
+
+/* Outputs, [nxp][nyp][nh][ns] host arrays; any may be NULL: the summed
+ * image km, its fold count, and the post-stack image (Kirchhoff.c:1423-1479). */
+typedef struct ktm_image {
+    float *km;
+    float *fold;
+    float *image;
+} ktm_image;
+
+int ktm_migrate(const ktm_config *cfg, const ktm_traces *traces, const ktm_velocity *vel,
+                ktm_image *out, ktm_stats *stats);
 
 /* ---- SEG-Y (host), READ_SEGY.c:29-115,169-450 and Kirchhoff.c:633-874 ---- */
 

@@ -76,6 +76,19 @@ class KtmStats(C.Structure):
                 ("total_ms", C.c_double)]
 
 
+class KtmTraces(C.Structure):
+    _fields_ = [("samples", C.c_void_p), ("coords", C.c_void_p), ("ntr", C.c_int64),
+                ("ns", C.c_int32), ("dt", C.c_float)]
+
+
+class KtmVelocity(C.Structure):
+    _fields_ = [("v", C.c_void_p), ("nsm", C.c_int32), ("dtm", C.c_float)]
+
+
+class KtmImage(C.Structure):
+    _fields_ = [("km", C.c_void_p), ("fold", C.c_void_p), ("image", C.c_void_p)]
+
+
 class KtmEomConfig(C.Structure):
     _fields_ = [("ox", C.c_double), ("oy", C.c_double), ("angle_e_to_n", C.c_double),
                 ("oxcsp", C.c_double), ("oycsp", C.c_double), ("excsp", C.c_double),
@@ -110,8 +123,8 @@ _PROTOS = {
     "ktm_image_reset": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "ktm_migrate_block": (_i32, [_vp, _vp, _i32, _i32, _vp, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp]),
     "ktm_poststack": (_i32, [_vp, _vp, _vp, _vp, _vp]),
-    "ktm_migrate": (_i32, [C.POINTER(KtmConfig), _i32, _f32, _vp, _vp, _i64, _vp, _i32, _f32,
-                           _vp, _vp, _vp, C.POINTER(KtmStats)]),
+    "ktm_migrate": (_i32, [C.POINTER(KtmConfig), C.POINTER(KtmTraces), C.POINTER(KtmVelocity),
+                           C.POINTER(KtmImage), C.POINTER(KtmStats)]),
     "ktm_ibm_to_float": (None, [_vp, _i64, _vp]),
     "ktm_segy_count": (_i32, [C.c_char_p, C.POINTER(_i32), C.POINTER(_f32), C.POINTER(_i32)]),
     "ktm_segy_read": (_i32, [C.c_char_p, _i64, _i64, _i32, _vp, _vp, _vp]),

@@ -370,9 +370,9 @@ static int run_survey(const ktm_plan *plan, const ktm_config *cfg, const char *p
     return st;
 }
 
-int ktm_migrate(const ktm_config *cfg, int ns, float dt, const float *traces, const int32_t *coords,
-                int64_t ntr, const float *vel, int nsm, float dtm, float *km, float *fold,
-                float *image, ktm_stats *stats)
+static int migrate_arrays(const ktm_config *cfg, int ns, float dt, const float *traces,
+                          const int32_t *coords, int64_t ntr, const float *vel, int nsm, float dtm,
+                          float *km, float *fold, float *image, ktm_stats *stats)
 {
     if (!cfg || ntr < 0 || (ntr > 0 && (!traces || !coords)) || !vel) return KTM_ERR_ARG;
     const double t_start = now_ms();
@@ -404,6 +404,15 @@ int ktm_migrate(const ktm_config *cfg, int ns, float dt, const float *traces, co
         stats->total_ms = now_ms() - t_start;
     }
     return st;
+}
+
+int ktm_migrate(const ktm_config *cfg, const ktm_traces *traces, const ktm_velocity *vel,
+                ktm_image *out, ktm_stats *stats)
+{
+    if (!traces || !vel) return KTM_ERR_ARG;
+    return migrate_arrays(cfg, traces->ns, traces->dt, traces->samples, traces->coords,
+                          traces->ntr, vel->v, vel->nsm, vel->dtm, out ? out->km : NULL,
+                          out ? out->fold : NULL, out ? out->image : NULL, stats);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -18,7 +18,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (ACCUM_KAHAN, ACCUM_PLAIN, PREP_REFERENCE, SEGY_IBM, TRACE_F32, KtmConfig,
-                   KtmSetup, KtmStats, check, lib)
+                   KtmImage, KtmSetup, KtmStats, KtmTraces, KtmVelocity, check, lib)
 
 
 @dataclass
@@ -256,9 +256,11 @@ def migrate_host(cfg: Config, ns: int, dt: float, traces: np.ndarray, coords: np
     st = KtmStats()
     c = cfg.to_c()
     vp = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
-    check(lib().ktm_migrate(C.byref(c), ns, dt, vp(traces), vp(coords), traces.shape[0], vp(vel),
-                            vel.shape[2], dt if dtm is None else dtm, vp(km), vp(fold), vp(image),
-                            C.byref(st)), "ktm_migrate")
+    tr = KtmTraces(vp(traces), vp(coords), traces.shape[0], ns, dt)
+    vm = KtmVelocity(vp(vel), vel.shape[2], dt if dtm is None else dtm)
+    out = KtmImage(vp(km), vp(fold), vp(image))
+    check(lib().ktm_migrate(C.byref(c), C.byref(tr), C.byref(vm), C.byref(out), C.byref(st)),
+          "ktm_migrate")
     return km, fold, image, {"contributions": st.contributions, "migrate_ms": st.migrate_ms,
                              "total_ms": st.total_ms}
 

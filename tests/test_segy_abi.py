@@ -46,6 +46,41 @@ def test_every_declared_function_has_a_ctypes_prototype():
     assert not missing, missing
 
 
+def test_migrate_argument_structs_match_the_header(tmp_path):
+    """ktm_traces / ktm_velocity / ktm_image / ktm_stats: the ctypes layouts
+    against gcc's (sizeof and offsetof of every field)."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    structs = {"ktm_traces": _lib.KtmTraces, "ktm_velocity": _lib.KtmVelocity,
+               "ktm_image": _lib.KtmImage, "ktm_stats": _lib.KtmStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ktm.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I" + os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                 text=True).stdout.splitlines())
+    for cname, py in structs.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, (cname, f)
+
+
+def test_migrate_rejects_missing_arguments():
+    L = _lib.lib()
+    c = Config().to_c()
+    st = _lib.KtmStats()
+    assert L.ktm_migrate(C.byref(c), None, None, None, C.byref(st)) == _lib.KTM_ERR_ARG
+
+
 def test_oracle_is_a_separate_library():
     # the product library does not contain (or link) the oracle
     L = _lib.lib()
