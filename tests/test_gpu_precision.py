@@ -128,22 +128,25 @@ def test_quantized_storage_matches_numpy_error_feedback(cuda, dtype):
     assert np.array_equal(got, want)
 
 
-@pytest.mark.parametrize("nh,jobs_x", [(1, 1), (4, 3)])
-def test_compensated_line_vs_fp64(cuda, nh, jobs_x):
+@pytest.mark.parametrize("nh,jobs_x,block", [(1, 1, None), (4, 3, None), (1, 1, 997)])
+def test_compensated_line_vs_fp64(cuda, nh, jobs_x, block):
     """The compensated migration of a 2-D line (8400 traces in one block,
     400 columns; four offset classes and three job slabs in the second
-    case): the contribution count and the fold equal the oracle's exactly
-    and the image is within 1e-6 of the FP64 oracle."""
+    case; blocks of 997 traces in the third, so the compensation image is
+    carried over nine launches and the staged batches end at every phase
+    of the kernel's four-batch compensated step): the contribution count
+    and the fold equal the oracle's exactly and the image is within 1e-6
+    of the FP64 oracle."""
     sv = surveys.line2d(nshots=70, ns=501, ncols=400, nscat=8, nh=nh, dh=1500.0 / nh,
                         jobs_x=jobs_x, t0_range=(0.3, 1.8))
     tr = surveys.traces_numpy(sv)
     km64, fold64, n64 = oracle.migrate(sv.cfg, sv.ns, sv.dt, tr, sv.coords, sv.vel,
                                        mode=oracle.FP64_DATA)
     print(f"case: compensated 2-D line, {sv.ntr} traces, 400 columns, nh {nh}, "
-          f"jobs_x {jobs_x}", flush=True)
+          f"jobs_x {jobs_x}, block {block}", flush=True)
     plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
     km, fold, n = plan.migrate(torch.from_numpy(tr).to(cuda), torch.from_numpy(sv.coords).to(cuda),
-                               mode=PREP_CLOSED, accum=ACCUM_KAHAN, block=sv.ntr)
+                               mode=PREP_CLOSED, accum=ACCUM_KAHAN, block=block or sv.ntr)
     km = km.cpu().numpy()
     r = rel_l2(km, km64)
     print(f"  relL2 {r:.3e}")
