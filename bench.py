@@ -56,12 +56,14 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the ingest, velocity-gradient, adjoint/LSM, post-stack and EOM legs")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3")
+    ap.add_argument("--profile-leg", default="fp32",
+                    help="with --profile: the precision leg (a PRECISION_LEGS label) to run")
     ap.add_argument("--leg-steps", type=int, default=3, help="timed blocks per precision leg")
     ap.add_argument("--err-traces", type=int, default=20000,
                     help="traces of the FP64-oracle error check on the 50x50x1500 sub-volume")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy 3x (HBM counter calibration) and exit")
-    ap.add_argument("--leg", choices=("vgrad", "adjoint", "poststack", "eom"),
+    ap.add_argument("--leg", choices=("vgrad", "adjoint", "poststack", "eom", "valu"),
                     help="only run this extra leg, print its JSON and exit (for profiling)")
     return ap.parse_args()
 
@@ -111,6 +113,58 @@ def measured_peaks(dev, reps=5):
     return copy_gbs, lds_gbs
 
 
+VALU_MODES = {0: "v_fma_f32", 1: "v_add_f32", 2: "v_sqrt_f32", 3: "v_cvt_u32_f32/v_cvt_f32_u32",
+              4: "v_fma_f32 dependent chain", 5: "migration tap mix (2 fma, 2 sqrt, add, min, cvt, "
+              "lshl_add, add)", 6: "1 v_sqrt_f32 : 4 v_fma_f32", 7: "1 v_sqrt_f32 : 1 v_fma_f32",
+              8: "v_min_f32", 9: "v_rndne_f32", 10: "v_lshl_add_u32",
+              11: "v_cmp_le_f32 (SGPR out; hazard nops: upper bound)", 12: "v_cndmask_b32",
+              13: "v_add_u32", 14: "v_fma_f64", 15: "v_readlane_b32 (hazard nops: upper bound)"}
+
+
+def valu_probe(dev, modes=tuple(VALU_MODES), waves_per_simd=(8,), target_ms=40.0):
+    """VALU issue rate of each probe mode (ktm_probe_valu): wave-instructions
+    per second over the whole chip and per CU-cycle at the shader clock the
+    probe itself recorded (clock64 against the 100 MHz wall clock)."""
+    import numpy as np
+    import torch
+
+    from random_amd import _lib
+    L = _lib.lib()
+    stream = torch.cuda.current_stream().cuda_stream
+    ncu = torch.cuda.get_device_properties(dev).multi_processor_count
+    out = {}
+    for wps in waves_per_simd:
+        nblocks = ncu * wps           # 4 waves per workgroup: wps waves per SIMD
+        buf = torch.zeros(3 * nblocks, dtype=torch.int32, device=dev)
+        for mode in modes:
+            ops = L.ktm_probe_valu_ops(mode)
+            iters = 256
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            for _ in range(6):        # grow iters until one launch takes ~target_ms
+                torch.cuda.synchronize()
+                e0.record()
+                _lib.check(L.ktm_probe_valu(mode, buf.data_ptr(), nblocks, iters, stream),
+                           "ktm_probe_valu")
+                e1.record()
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1)
+                if ms >= 0.5 * target_ms:
+                    break
+                iters = min(1 << 24, int(iters * min(64.0, target_ms / max(ms, 1e-3))))
+            w = buf.cpu().numpy().view(np.uint32).reshape(nblocks, 3).astype(np.float64)
+            ghz = float(np.median(w[:, 0] / w[:, 1]) * 0.1)
+            insts = float(nblocks) * 4 * iters * ops
+            rate = insts / (ms * 1e-3)
+            out[f"{mode}@{wps}"] = {"mode": VALU_MODES[mode], "waves_per_simd": wps,
+                                    "wave_insts": insts, "ms": round(ms, 3),
+                                    "wave_insts_per_s": rate, "shader_clock_ghz": round(ghz, 3),
+                                    "per_cu_cycle": round(rate / (ncu * ghz * 1e9), 4)}
+            log(f"valu probe mode {mode} ({VALU_MODES[mode]}), {wps} waves/SIMD: {ms:.2f} ms, "
+                f"{rate:.4g} wave-insts/s, clock {ghz:.3f} GHz, "
+                f"{out[f'{mode}@{wps}']['per_cu_cycle']:.3f} per CU-cycle")
+    return out
+
+
 # ---- CPU-baseline leg: the only code in this file that runs the oracle
 # (oracle/), as the CPU timing beside the GPU figures and as the checker of
 # the GPU precision modes; every GPU figure comes from libktm.so. ----
@@ -151,18 +205,31 @@ def cpu_baseline(sv, traces_np, target_s, nthreads):
                                       f"-ffast-math (Kirchhoff.c:3), {el4:.1f} s"}
 
 
-def cpu_baseline_fp64(sv, cfg, tr_np, co_np):
-    """CPU-baseline leg, checker part: the FP64 oracle image (FP32
-    traveltime index, FP64 data path) of a sub-volume, against which the
-    GPU precision modes are measured.  Returns (km64, contributions)."""
+def cpu_baseline_subvolume(sv, cfg, tr_np, co_np):
+    """CPU-baseline leg, checker part: the oracle on a sub-volume, against which
+    the GPU paths are measured -- its FP32 mode (the reference algorithm: the
+    GPU's FP32 reference path must equal it bit for bit), FP64_DATA (the
+    reference's FP32 traveltime index, FP64 data path: what storage and
+    accumulation error is measured against), FP64_FULL (every float of the
+    loop widened, the traveltime too), and the FP64 post-stack image of the
+    FP64_DATA sums.  Returns a dict of arrays and counts."""
     import oracle
     nthreads = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
+    out = {}
     t0 = time.perf_counter()
-    km64, _, n64 = oracle.migrate(cfg, sv.ns, sv.dt, tr_np, co_np, sv.vel, mode=oracle.FP64_DATA,
-                                  nthreads=nthreads)
-    log(f"fp64 oracle on the sub-volume: {n64:.3e} contributions in "
-        f"{time.perf_counter() - t0:.1f}s")
-    return km64, n64
+    out["km32"], out["fold32"], out["n32"] = oracle.migrate(cfg, sv.ns, sv.dt, tr_np, co_np, sv.vel,
+                                                            mode=oracle.FP32, nthreads=nthreads)
+    out["km64"], out["fold64"], out["n64"] = oracle.migrate(cfg, sv.ns, sv.dt, tr_np, co_np, sv.vel,
+                                                            mode=oracle.FP64_DATA,
+                                                            nthreads=nthreads)
+    out["km64f"], out["fold64f"], _ = oracle.migrate(cfg, sv.ns, sv.dt, tr_np, co_np, sv.vel,
+                                                     mode=oracle.FP64_FULL, nthreads=nthreads)
+    _, aa = oracle.alias_filter(cfg, sv.ns, sv.dt)
+    out["img64"] = oracle.poststack(out["km64"], out["fold64"], sv.dt, aa)
+    out["img64f"] = oracle.poststack(out["km64f"], out["fold64f"], sv.dt, aa)
+    log(f"oracle on the sub-volume (FP32, FP64_DATA, FP64_FULL, post-stack): {out['n64']:.3e} "
+        f"contributions, {time.perf_counter() - t0:.1f}s")
+    return out
 
 
 def cpu_baseline_eom(cfg, sv, tr, coords):
@@ -178,37 +245,128 @@ def cpu_baseline_eom(cfg, sv, tr, coords):
 # ---- end of the CPU-baseline leg ----
 
 
-def precision_errors(sv, plan, traces, coords, ntr, dev):
-    """Errors of every path against the FP64 oracle (FP32 traveltime index,
-    FP64 data path) on the 50 x 50 x 1500 sub-volume at columns 175..224 of
-    both axes, migrating the first ntr traces of the survey."""
-    import numpy as np
-    import torch
+PRECISION_LEGS = (
+    # label, prep mode, trace storage, accumulation | index flags
+    ("fp32", "REFERENCE", "F32", "PLAIN"),
+    ("fp32_closed_kahan", "CLOSED", "F32", "KAHAN"),
+    ("fp16", "CLOSED", "F16", "KAHAN"),
+    ("bf16", "CLOSED", "BF16", "KAHAN"),
+    ("bf16_plain", "CLOSED", "BF16", "PLAIN"),
+    ("fp32_closed_fast", "CLOSED", "F32", "PLAIN|FAST"),
+    ("fp32_closed_kahan_fast", "CLOSED", "F32", "KAHAN|FAST"),
+    ("fp16_fast", "CLOSED", "F16", "KAHAN|FAST"),
+    ("bf16_fast", "CLOSED", "BF16", "KAHAN|FAST"),
+)
 
-    from random_amd import (ACCUM_KAHAN, ACCUM_PLAIN, PREP_CLOSED, PREP_REFERENCE, TRACE_BF16,
-                            TRACE_F16, TRACE_F32)
+
+def leg_args(prep, store, acc):
+    """(prep mode, trace dtype, accum flags) of a PRECISION_LEGS row."""
+    import random_amd as ra
+    flags = 0
+    for f in acc.split("|"):
+        flags |= {"PLAIN": ra.ACCUM_PLAIN, "KAHAN": ra.ACCUM_KAHAN, "FAST": ra.INDEX_FAST}[f]
+    return (getattr(ra, f"PREP_{prep}"), getattr(ra, f"TRACE_{store}"), flags)
+
+
+def precision_errors(sv, plan, traces, coords, ntr, dev):
+    """Every path on the 50 x 50 x ns sub-volume at columns 175..224 of both
+    axes, migrating the first ntr traces of the survey: the FP32 reference
+    path against the oracle's FP32 mode (bit for bit, fold and contribution
+    count), and every path's image sum km and post-stack image against the
+    FP64 oracle with the reference's FP32 traveltime index (FP64_DATA) and
+    with an FP64 traveltime (FP64_FULL)."""
+    import numpy as np
+
     from random_amd.ktm import Plan
     cfg = sv.cfg.replace(start_iline=175, end_iline=225, start_xline=175, end_xline=225,
                          jobs_x=10, jobs_y=2)
     sub = Plan(cfg, sv.ns, sv.dt, sv.vel)
-    km64, n64 = cpu_baseline_fp64(sv, cfg, traces[:ntr].cpu().numpy(), sv.coords[:ntr])
+    ref = cpu_baseline_subvolume(sv, cfg, traces[:ntr].cpu().numpy(), sv.coords[:ntr])
     out = {"error_check": f"50x50x{sv.ns} sub-volume (columns 175-224 x 175-224), first {ntr} "
-                          f"traces, {n64:.3e} contributions; reference = CPU oracle with FP64 data "
-                          f"path and the reference FP32 traveltime index"}
-    nrm = float(np.linalg.norm(km64))
-    mx = float(np.abs(km64).max())
-    for label, mode, dtype, accum in (
-            ("fp32", PREP_REFERENCE, TRACE_F32, ACCUM_PLAIN),
-            ("fp32_closed_kahan", PREP_CLOSED, TRACE_F32, ACCUM_KAHAN),
-            ("fp16", PREP_CLOSED, TRACE_F16, ACCUM_KAHAN),
-            ("bf16", PREP_CLOSED, TRACE_BF16, ACCUM_KAHAN),
-            ("bf16_plain", PREP_CLOSED, TRACE_BF16, ACCUM_PLAIN)):
-        km, _, n = sub.migrate(traces[:ntr], coords[:ntr], mode=mode, dtype=dtype, accum=accum)
-        d = km.cpu().numpy().astype(np.float64) - km64
-        e = {"rel_l2_vs_fp64": float(np.linalg.norm(d) / nrm),
-             "max_abs_vs_fp64_rel_to_max": float(np.abs(d).max() / mx)}
+                          f"traces, {ref['n64']:.3e} contributions; FP64_DATA = oracle with the "
+                          f"reference FP32 traveltime index and an FP64 data path, FP64_FULL = "
+                          f"every float of the loop widened (FP64 traveltime); image = post-stack "
+                          f"output (Kirchhoff.c:1423-1479), km = the migrated sum before it"}
+    n64 = np.linalg.norm(ref["km64"])
+    m64 = float(np.abs(ref["km64"]).max())
+    n64f = np.linalg.norm(ref["km64f"])
+    ni64 = np.linalg.norm(ref["img64"])
+    ni64f = np.linalg.norm(ref["img64f"])
+    for label, prep, store, acc in PRECISION_LEGS:
+        mode, dtype, accum = leg_args(prep, store, acc)
+        km, fold, n = sub.migrate(traces[:ntr], coords[:ntr], mode=mode, dtype=dtype, accum=accum)
+        img = sub.poststack(km, fold).cpu().numpy().astype(np.float64)
+        k = km.cpu().numpy()
+        d = k.astype(np.float64) - ref["km64"]
+        e = {"rel_l2_vs_fp64": float(np.linalg.norm(d) / n64),
+             "max_abs_vs_fp64_rel_to_max": float(np.abs(d).max() / m64),
+             "rel_l2_vs_fp64_full": float(np.linalg.norm(k - ref["km64f"]) / n64f),
+             "image_rel_l2_vs_fp64": float(np.linalg.norm(img - ref["img64"]) / ni64),
+             "image_rel_l2_vs_fp64_full": float(np.linalg.norm(img - ref["img64f"]) / ni64f)}
+        if label == "fp32":
+            f = fold.cpu().numpy()
+            e["vs_oracle_fp32"] = {
+                "km_bit_identical_fraction": float(np.mean(k.view(np.uint32) ==
+                                                           ref["km32"].view(np.uint32))),
+                "fold_equal": bool(np.array_equal(f, ref["fold32"])),
+                "contributions": n, "contributions_oracle": ref["n32"],
+                "contributions_equal": bool(n == ref["n32"])}
         out.setdefault(label, {}).update(e)
         log(f"{label}: {e}")
+    return out
+
+
+def index_mismatch(sv, plan, geom, dev, nsamp=1 << 22, seed=5):
+    """How often the fast form's floor(q) differs from the reference's
+    traveltime index, on (trace, column, sample) triples drawn from the
+    workload: random traces of a block, random image columns within the CMP
+    aperture, uniform samples; q and the reference index come from the
+    traveltime self-test kernel (ktm_selftest_traveltime)."""
+    import numpy as np
+    import torch
+
+    from random_amd import _lib
+    rng = np.random.default_rng(seed)
+    g = geom.cpu().numpy().view(np.float32).reshape(-1, 8)
+    ns, dt = sv.ns, np.float32(sv.dt)
+    ncol = sv.cfg.end_iline - sv.cfg.start_iline
+    d = sv.cfg.iline_distance
+    ap2 = np.float32(sv.cfg.aperture) ** 2
+    rows = []
+    have = 0
+    while have < nsamp:
+        i = rng.integers(0, g.shape[0], nsamp)
+        X = rng.integers(0, ncol, nsamp)
+        Y = rng.integers(0, ncol, nsamp)
+        IX = (X * d).astype(np.float32)
+        IY = (Y * d).astype(np.float32)
+        gi = g[i]
+        ok = (IX - gi[:, 4]) ** 2 + (IY - gi[:, 5]) ** 2 < ap2
+        t = rng.integers(0, ns, nsamp)
+        dis2 = (IX - gi[:, 0]) ** 2 + (IY - gi[:, 1]) ** 2
+        dir2 = (IX - gi[:, 2]) ** 2 + (IY - gi[:, 3]) ** 2
+        v2 = sv.vel[X, Y, t] * sv.vel[X, Y, t]
+        rows.append(np.stack([dis2, dir2, t.astype(np.float32), v2], 1)[ok])
+        have += int(ok.sum())
+    inp = np.ascontiguousarray(np.concatenate(rows)[:nsamp], np.float32)
+    d_in = torch.from_numpy(inp).to(dev)
+    n = inp.shape[0]
+    qf = torch.empty(n, dtype=torch.float32, device=dev)
+    qr = torch.empty_like(qf)
+    tk = torch.empty(n, dtype=torch.int32, device=dev)
+    tr = torch.empty_like(tk)
+    _lib.check(_lib.lib().ktm_selftest_traveltime(d_in.data_ptr(), n, float(dt), ns, qf.data_ptr(),
+                                                  qr.data_ptr(), tk.data_ptr(), tr.data_ptr(),
+                                                  torch.cuda.current_stream().cuda_stream),
+               "ktm_selftest_traveltime")
+    q = qf.cpu().numpy()
+    t_fast = np.where(q < ns, np.floor(q), ns).astype(np.int64)
+    t_ref = tr.cpu().numpy().astype(np.int64)
+    t_ker = tk.cpu().numpy().astype(np.int64)
+    mism = t_fast != t_ref
+    out = {"samples": int(n), "fast_form_mismatch_rate": float(mism.mean()),
+           "exact_path_mismatch_rate": float(np.mean(t_ker != t_ref))}
+    log(f"index mismatch: {out}")
     return out
 
 
@@ -495,7 +653,8 @@ def main():
     if args.leg:
         import torch
         fn = {"vgrad": vgrad_leg, "adjoint": adjoint_leg, "poststack": poststack_leg,
-              "eom": eom_leg}[args.leg]
+              "eom": eom_leg,
+              "valu": lambda d: valu_probe(d, waves_per_simd=(1, 2, 4, 8))}[args.leg]
         print(json.dumps(fn(torch.device("cuda:0"))), flush=True)
         return
     if args.gpus != 1:
@@ -506,8 +665,8 @@ def main():
     import numpy as np
     import torch
 
-    from random_amd import (ACCUM_KAHAN, ACCUM_PLAIN, PREP_CLOSED, PREP_REFERENCE, TRACE_BF16,
-                            TRACE_F16, TRACE_F32, surveys, synth)
+    from random_amd import (ACCUM_KAHAN, ACCUM_PLAIN, INDEX_FAST, PREP_CLOSED, PREP_REFERENCE,
+                            TRACE_BF16, TRACE_F16, TRACE_F32, surveys, synth)
     from random_amd.ktm import Plan
 
     dev = torch.device("cuda:0")
@@ -528,7 +687,7 @@ def main():
     def run_mode(steps, warm, mode, dtype, accum, label):
         """warm + steps trace blocks into a fresh image; returns (G contrib/s of
         the timed steps incl. prep, per-launch migrate ms list, contributions)."""
-        km, fold, comp = plan.new_image(kahan=accum == ACCUM_KAHAN)
+        km, fold, comp = plan.new_image(kahan=(accum & ~INDEX_FAST) == ACCUM_KAHAN)
         counter = torch.zeros(1, dtype=torch.int64, device=dev)
 
         def run_block(b):
@@ -567,6 +726,13 @@ def main():
         return n / (total_ms * 1e-3) / 1e9, mig_ms, n, total_ms
 
     steps, warm = args.steps, args.warmup
+    if args.profile and args.profile_leg != "fp32":
+        row = [r for r in PRECISION_LEGS if r[0] == args.profile_leg][0]
+        mode, dtype, accum = leg_args(*row[1:4])
+        g, ms, n, tot = run_mode(steps, warm, mode, dtype, accum, row[0])
+        print(json.dumps({"leg": row[0], "value": round(g, 3), "avg_launch_ms": float(np.mean(ms)),
+                          "contributions": n}), flush=True)
+        return
     gcs, mig_ms, n_contrib, total_ms = run_mode(steps, warm, PREP_REFERENCE, TRACE_F32,
                                                 ACCUM_PLAIN, "fp32")
     per_launch = n_contrib / steps
@@ -603,17 +769,18 @@ def main():
                     "unit": "GB/s"}})
     if not args.profile and args.leg_steps > 0:
         legs = {}
-        for label, mode, dtype, accum, bpc in (
-                ("bf16", PREP_CLOSED, TRACE_BF16, ACCUM_KAHAN, BYTES_HALF),
-                ("fp16", PREP_CLOSED, TRACE_F16, ACCUM_KAHAN, BYTES_HALF),
-                ("bf16_plain", PREP_CLOSED, TRACE_BF16, ACCUM_PLAIN, BYTES_HALF),
-                ("fp32_closed_kahan", PREP_CLOSED, TRACE_F32, ACCUM_KAHAN, BYTES_FP32)):
+        for label, prep, store, acc in PRECISION_LEGS:
+            if label == "fp32":
+                continue
+            mode, dtype, accum = leg_args(prep, store, acc)
             g, ms, n, _ = run_mode(args.leg_steps, 1, mode, dtype, accum, label)
             legs[label] = {"value": round(g, 3), "speedup_vs_fp32": round(g / gcs, 3),
                            "avg_launch_ms": round(float(np.mean(ms)), 2),
-                           "roofline_frac": round(n / len(ms) * bpc / (np.mean(ms) * 1e-3) / 1e9
-                                                  / HBM_PEAK_GBS, 4),
-                           "bytes_per_contribution": bpc}
+                           "trace_bytes_resident": int(sv.ntr * plan.row *
+                                                       (2 if store in ("BF16", "F16") else 4))}
+        blk, geom0 = plan.prep(traces[:args.block], coords[:args.block], PREP_CLOSED, TRACE_F32)
+        legs["index_mismatch"] = index_mismatch(sv, plan, geom0, dev)
+        del blk, geom0
         if not args.no_cpu and args.err_traces > 0:
             for k, v in precision_errors(sv, plan, traces, coords, args.err_traces, dev).items():
                 if isinstance(v, dict):

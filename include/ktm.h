@@ -109,6 +109,15 @@ enum { KTM_TRACE_F32 = 0, KTM_TRACE_BF16 = 1, KTM_TRACE_F16 = 2 };
  * traces are summed in FP32, then added with one compensated step, so the
  * error stays of order 2^-24 sum |tap|). */
 enum { KTM_ACCUM_PLAIN = 0, KTM_ACCUM_KAHAN = 1 };
+/* OR'ed into the accumulation argument of ktm_migrate_block with CLOSED prep
+ * (FP32, BF16 or FP16 storage): the traveltime index is floor(q) of the
+ * single-precision fast form q = sqrt(dIS2/(v^2 dt^2) + t^2/4) +
+ * sqrt(dIR2/(v^2 dt^2) + t^2/4) (hardware square roots, one FMA per
+ * radicand), without the reference-exact fix-up of the samples whose q lies
+ * within 7 ulp of an integer (Kirchhoff.c:1381-1384 rounds differently
+ * there).  No counterpart in the reference; for the modes whose contract is
+ * an error bound against FP64 rather than the reference's bits. */
+enum { KTM_INDEX_FAST = 0x100 };
 
 /* Device-resident setup: velocity model (squared, Kirchhoff.c:617), traveltime
  * tables and the image-column tables of one configuration.  Samples per
@@ -468,6 +477,18 @@ int ktm_synth_noise(float *traces, int64_t n, float sigma, uint64_t seed, void *
  * of nblocks workgroups of 256 lanes, each reading iters * 64 bytes per lane
  * from LDS; out (device) receives nblocks * 256 words. */
 int ktm_probe_lds(uint32_t *out, int nblocks, int iters, void *stream);
+/* Measurement helper (no reference counterpart): a VALU issue-rate probe.
+ * nblocks workgroups of 256 lanes each run iters iterations of a fixed
+ * inline-assembly VALU sequence over 8 independent register chains: mode 0
+ * v_fma_f32, 1 v_add_f32, 2 v_sqrt_f32, 3 v_cvt_u32_f32/v_cvt_f32_u32,
+ * 4 one dependent v_fma_f32 chain, 5 the migration tap (2 fma, 2 sqrt, add,
+ * min, cvt, lshl_add, add), 6 one sqrt per four fma, 7 one sqrt per fma.
+ * ktm_probe_valu_ops(mode) is the VALU instruction count per lane and
+ * iteration (-1 for a bad mode).  out (device) receives 3 words per
+ * workgroup: shader-clock cycles (clock64) and 100 MHz ticks (wall_clock64)
+ * of its loop, and a checksum. */
+int ktm_probe_valu(int mode, uint32_t *out, int nblocks, int iters, void *stream);
+int ktm_probe_valu_ops(int mode);
 
 /* The float t such that, for every float x >= 0, x < t exactly when
  * (double)(float)sqrt((double)x) < A -- the form of the aperture and slab

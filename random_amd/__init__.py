@@ -8,11 +8,11 @@ mirror of the reference's macros, device-resident plans driven with torch
 tensors, seeded synthetic surveys, the adjoint pair and least-squares
 migration.
 """
-from ._lib import (ACCUM_KAHAN, ACCUM_PLAIN, PREP_CLOSED, PREP_REFERENCE, TRACE_BF16,
-                   TRACE_F16, TRACE_F32, KtmError)
+from ._lib import (ACCUM_KAHAN, ACCUM_PLAIN, INDEX_FAST, PREP_CLOSED, PREP_REFERENCE,
+                   TRACE_BF16, TRACE_F16, TRACE_F32, KtmError)
 from .ktm import Config, Plan, migrate_host, setup_info
 
 __all__ = [
-    "ACCUM_KAHAN", "ACCUM_PLAIN", "PREP_CLOSED", "PREP_REFERENCE", "TRACE_BF16", "TRACE_F16",
+    "ACCUM_KAHAN", "ACCUM_PLAIN", "INDEX_FAST", "PREP_CLOSED", "PREP_REFERENCE", "TRACE_BF16", "TRACE_F16",
     "TRACE_F32", "KtmError", "Config", "Plan", "migrate_host", "setup_info",
 ]

@@ -29,6 +29,7 @@ STATUS_NAMES = {
 PREP_REFERENCE, PREP_CLOSED = 0, 1
 TRACE_F32, TRACE_BF16, TRACE_F16 = 0, 1, 2
 ACCUM_PLAIN, ACCUM_KAHAN = 0, 1
+INDEX_FAST = 0x100          # OR'ed into accum: relaxed traveltime index (CLOSED prep)
 SEGY_IBM, SEGY_IEEE = 1, 5          # SEG-Y sample format codes
 
 
@@ -157,6 +158,8 @@ _PROTOS = {
     "ktm_eom_block": (_i32, [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp]),
     "ktm_eom_post": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "ktm_probe_lds": (_i32, [_vp, _i32, _i32, _vp]),
+    "ktm_probe_valu": (_i32, [_i32, _vp, _i32, _i32, _vp]),
+    "ktm_probe_valu_ops": (_i32, [_i32]),
     "ktm_eom_job_size": (_i64, [_vp]),
     "ktm_eom_job_reset": (_i32, [_vp, _vp, _vp, _vp]),
     "ktm_eom_job_block": (_i32, [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp]),

@@ -600,6 +600,14 @@ __device__ __forceinline__ int fast_T(float qc) { return (int)qc; }
  * compensated step */
 #define KTM_KAHAN_FOLD 4
 
+/* q clamped to [0, hi] in one instruction (v_med3_f32; q >= 0 is a sum of
+ * square roots, so this is min(q, hi) without the canonicalisation of
+ * fminf's operands, which the compiler otherwise repeats per sample) */
+__device__ __forceinline__ float clamp_ns(float q, float hi)
+{
+    return __builtin_amdgcn_fmed3f(q, 0.0f, hi);
+}
+
 /* One workgroup = 8 waves = NCOL image columns x W time segments.  Each lane
  * owns K samples t = seg*64K + 64k + lane of its column: the image values
  * (km, fold count, Kahan compensation), t2[t] and 1/v^2 stay in registers
@@ -608,9 +616,22 @@ __device__ __forceinline__ int fast_T(float qc) { return (int)qc; }
  * (ordered compaction), then staged 8 at a time; per staged trace the
  * aperture and slab gates of every column are evaluated lane-parallel
  * (one lane per trace) and the passing traces are walked in order.
+ *
+ * FASTIDX = false: the reference's traveltime index, exactly: the fast form
+ *   plus the reference arithmetic (exact_T) wherever an integer lies within
+ *   the fast form's error bound of q.  q is clamped at ns, so a sample at or
+ *   past the end of the trace lands on an integer and takes that rare path,
+ *   where the samples that do not count towards fold are counted: the
+ *   common path has no compare for fold.
+ * FASTIDX = true: T = floor(min(q, ns)) of the fast form alone, for the
+ *   CLOSED and 16-bit modes whose contract is an error bound against FP64
+ *   rather than the reference's bits: no ambiguity test, no exact path, no
+ *   v^2 in LDS.  Fold is summed from a row V[T] = (T < ns) staged once in
+ *   LDS (an address add and a float add per sample instead of a compare
+ *   and a carry add, both quarter-rate on gfx950).
  * The plain variants fit 128 VGPRs (4 waves per SIMD, two workgroups per CU,
  * one staging while the other computes). */
-template <int K, int W, int B, bool KAHAN>
+template <int K, int W, int B, bool KAHAN, bool FASTIDX>
 __global__ void __launch_bounds__(8 * KTM_WAVE) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_migrate(const MigArgs a)
 {
@@ -623,7 +644,9 @@ k_migrate(const MigArgs a)
     float *sG = sF + B * ldl;                               /* [B][8]   */
     int *sSel = reinterpret_cast<int *>(sG + B * 8);        /* [NT]     */
     int *sCnt = sSel + NT;                                  /* [NW]     */
-    float *sV2 = reinterpret_cast<float *>(sCnt + NW);      /* [NCOL][W*K*64] v^2 per sample */
+    /* exact index: v^2 per sample, [NCOL][W*K*64]; fast index: V[T] = (T < ns),
+     * [ldl] */
+    float *sV2 = reinterpret_cast<float *>(sCnt + NW);
 
     const int tid = threadIdx.x, lane = tid & (KTM_WAVE - 1), wv = tid / KTM_WAVE;
     /* XCD-aware tile order: consecutive workgroup ids go to different XCDs.
@@ -659,37 +682,41 @@ k_migrate(const MigArgs a)
     const float reach = a.aperture * 1.0001f + sqrtf(hx * hx + hy * hy) + 1.0f;
     const float reach2 = reach * reach;
 
-
     /* v^2 of this column for the exact path lives in LDS (t2 is read from
      * the table there); the fast path (fast_qs: radicands pre-scaled by
      * 1/dt^2) keeps rs = RN(1/(v^2 dt^2)) and tq = t^2/4 in registers.  The
-     * plain variants keep RN(1/v^2) for the exact path in registers too; the
-     * Kahan variants, whose compensation takes K registers, divide for it on
-     * the exact path. */
+     * plain exact-index variants keep RN(1/v^2) for the exact path in
+     * registers too; the Kahan variants, whose compensation takes K
+     * registers, divide for it on the exact path. */
     float *sV2c = sV2 + cw * (W * K * KTM_WAVE);
+    if (FASTIDX) {
+        for (int c = tid; c < ldl; c += NT) sV2[c] = c < ns ? 1.0f : 0.0f;
+    }
     const int64_t pix = (((int64_t)Xc * a.nyp + Yc) * a.nh + h) * ns;
     const int64_t vbase = ((int64_t)a.velx[Xc] * a.nvel_xl + a.vely[Yc]) * a.nsm;
-    constexpr bool RREG = !KAHAN;           /* RN(1/v^2) per slot in registers */
+    constexpr bool RREG = !KAHAN && !FASTIDX;   /* RN(1/v^2) per slot in registers */
     float r[RREG ? K : 1], rs[K], tq[K], acc[K], cmp[K];
-    int cnt[K];                             /* passing traces with T >= ns, per slot */
+    int cnt[FASTIDX ? 1 : K];               /* passing traces with T >= ns, per slot */
+    float vcnt[FASTIDX ? K : 1];            /* passing traces with T < ns, per slot */
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int tl = seg * K * KTM_WAVE + k * KTM_WAVE + lane;
-        cnt[k] = 0;
+        if (!FASTIDX) cnt[FASTIDX ? 0 : k] = 0;
+        if (FASTIDX) vcnt[FASTIDX ? k : 0] = 0.0f;
         cmp[k] = 0.0f;
         if (tl < ns) {
             const float v2 = a.vel2[vbase + a.t1[tl]];
-            sV2c[tl] = v2;
-            if (RREG) r[k] = 1.0f / v2;
+            if (!FASTIDX) sV2c[tl] = v2;
+            if (RREG) r[RREG ? k : 0] = 1.0f / v2;
             rs[k] = fast_rs(v2, a.dt);
             tq[k] = fast_tq(tl);
             acc[k] = col_ok ? a.km[pix + tl] : 0.0f;
             if (KAHAN && col_ok) cmp[k] = a.comp[pix + tl];
         } else {
-            sV2c[tl] = 1.0f;
+            if (!FASTIDX) sV2c[tl] = 1.0f;
             /* padding lane: q = 0.5, never flagged by the ambiguity test; it
-             * reads F[0] into a value that is not stored */
-            if (RREG) r[k] = 0.0f;
+             * reads F[0] (and V[0]) into values that are not stored */
+            if (RREG) r[RREG ? k : 0] = 0.0f;
             rs[k] = 0.0f;
             tq[k] = 0.0625f;
             acc[k] = 0.0f;
@@ -777,29 +804,28 @@ k_migrate(const MigArgs a)
                 uint64_t pm = __ballot(gp);
                 npass += __popcll(pm);
                 /* Hot loop, Kirchhoff.c:1374-1399: the tap F[T] of slot k for
-                 * one trace.  Fast path (fast_q) with the reference arithmetic
-                 * (exact_T) wherever an integer is within the fast path's
-                 * error bound of q, so T is always the reference's. */
+                 * one trace (and, fast index, V[T] for fold). */
                 auto tap = [&](int k, float dIS2, float dIR2, const float *Fr) __attribute__((always_inline)) {
                     const float q = fast_qs(dIS2, dIR2, rs[k], tq[k]);
-                    /* clamped at ns: a sample at or past the end of the
-                     * trace lands on an integer and takes the rare path
-                     * with the ambiguous ones, where the samples that do
-                     * not count towards fold are counted, so the common
-                     * path has no compare for fold */
-                    constexpr float EPS = KTM_FAST_EPS_S;
-                    const float qc = fminf(q, nsf);
+                    const float qc = clamp_ns(q, nsf);
                     int T = fast_T(qc);
+                    if (FASTIDX) {
+                        /* V sits (sV2 - Fr) floats from F: one address add */
+                        const float *Fp = Fr + T;
+                        vcnt[FASTIDX ? k : 0] += Fp[sV2 - Fr];
+                        return *Fp;
+                    }
+                    constexpr float EPS = KTM_FAST_EPS_S;
                     if (fast_ambiguous(qc, EPS)) {
                         if (q - nsf >= __builtin_fmaf(q, EPS, EPS))
                             T = ns;   /* Q >= ns as well: past the trace */
                         else {
                             const int tl = seg * K * KTM_WAVE + k * KTM_WAVE + lane;
                             const float v2 = sV2c[tl];
-                            T = exact_T(dIS2, dIR2, a.t2[tl], v2, RREG ? r[RREG ? k : 0] : 1.0f / v2,
-                                        a.dt, a.rdt, ns);
+                            T = exact_T(dIS2, dIR2, a.t2[tl], v2,
+                                        RREG ? r[RREG ? k : 0] : 1.0f / v2, a.dt, a.rdt, ns);
                         }
-                        cnt[k] += T >= ns ? 1 : 0;
+                        cnt[FASTIDX ? 0 : k] += T >= ns ? 1 : 0;
                     }
                     return Fr[T];
                 };
@@ -833,8 +859,10 @@ k_migrate(const MigArgs a)
             if (t < ns) {
                 a.km[pix + t] = acc[k];
                 /* fold adds the passing traces whose sample was inside the
-                 * trace (Kirchhoff.c:1387-1396): all of them minus cnt */
-                a.fold[pix + t] = a.fold[pix + t] + (float)((int)npass - cnt[k]);
+                 * trace (Kirchhoff.c:1387-1396) */
+                a.fold[pix + t] = a.fold[pix + t] +
+                                  (FASTIDX ? vcnt[FASTIDX ? k : 0]
+                                           : (float)((int)npass - cnt[FASTIDX ? 0 : k]));
                 if (KAHAN) a.comp[pix + t] = -y[KAHAN ? k : 0];
             }
         }
@@ -1462,6 +1490,7 @@ extern "C" int ktm_plan_setup(const ktm_plan *p, ktm_setup *out)
 
 extern "C" int64_t ktm_plan_trace_row(const ktm_plan *p) { return p ? p->row : 0; }
 
+
 extern "C" int64_t ktm_plan_image_size(const ktm_plan *p)
 {
     if (!p) return 0;
@@ -1820,12 +1849,16 @@ extern "C" int ktm_image_reset(const ktm_plan *p, float *km, float *fold, float 
 /* kernel choice: K sample slots per lane and W waves per column cover ns */
 typedef void (*mig_fn)(const MigArgs);
 
-/* B traces staged per batch (8: two workgroups per CU fit the LDS) */
+/* B traces staged per batch (8: two workgroups per CU fit the LDS); accum
+ * is KTM_ACCUM_PLAIN or KTM_ACCUM_KAHAN, optionally with KTM_INDEX_FAST */
 template <int K, int W, int B_ = 8>
 static mig_fn pick_accum(int accum, int *B)
 {
     *B = B_;
-    return accum == KTM_ACCUM_KAHAN ? k_migrate<K, W, B_, true> : k_migrate<K, W, B_, false>;
+    const bool kahan = (accum & ~KTM_INDEX_FAST) == KTM_ACCUM_KAHAN;
+    if (accum & KTM_INDEX_FAST)
+        return kahan ? k_migrate<K, W, B_, true, true> : k_migrate<K, W, B_, false, true>;
+    return kahan ? k_migrate<K, W, B_, true, false> : k_migrate<K, W, B_, false, false>;
 }
 
 /* W waves share one image column (each owns K*64 consecutive samples);
@@ -1897,10 +1930,14 @@ extern "C" int ktm_migrate_block(const ktm_plan *p, const void *prepared, int dt
     if (!p || !km || !fold || ntr < 0) return KTM_ERR_ARG;
     if (!ntr) return KTM_OK;
     if (!prepared || !geom) return KTM_ERR_ARG;
-    if (accum != KTM_ACCUM_PLAIN && accum != KTM_ACCUM_KAHAN) return KTM_ERR_ARG;
-    if (accum == KTM_ACCUM_KAHAN && !comp) return KTM_ERR_ARG;
+    const int acc_mode = accum & ~KTM_INDEX_FAST;
+    if (acc_mode != KTM_ACCUM_PLAIN && acc_mode != KTM_ACCUM_KAHAN) return KTM_ERR_ARG;
+    if (acc_mode == KTM_ACCUM_KAHAN && !comp) return KTM_ERR_ARG;
     if (dtype != KTM_TRACE_F32 && dtype != KTM_TRACE_BF16 && dtype != KTM_TRACE_F16) return KTM_ERR_ARG;
     if (dtype != KTM_TRACE_F32 && prep_mode != KTM_PREP_CLOSED) return KTM_ERR_UNSUPPORTED;
+    /* the relaxed index is an option of the CLOSED operator only: the FP32
+     * reference prep needs the reference's index (its taps cancel) */
+    if ((accum & KTM_INDEX_FAST) && prep_mode != KTM_PREP_CLOSED) return KTM_ERR_UNSUPPORTED;
     const ktm_setup *s = &p->tb.s;
     if (job >= s->no_job) return KTM_ERR_ARG;
     hipStream_t hs = (hipStream_t)stream;
@@ -1940,8 +1977,12 @@ extern "C" int ktm_migrate_block(const ktm_plan *p, const void *prepared, int dt
     const int tiles_y = (a.ry1 - a.ry0 + a.ty_cols - 1) / a.ty_cols;
     a.ntiles = a.tiles_x * tiles_y;
     const int nthreads = 8 * KTM_WAVE;
+    /* staged rows + geometry, selection list and counts, and v^2 per sample
+     * (exact index) or the fold row V (fast index) */
     const size_t lds = sizeof(float) * (size_t)(B * p->row + B * 8) +
-                       sizeof(int) * (size_t)(nthreads + 8) + sizeof(float) * (size_t)(8 * K * KTM_WAVE);
+                       sizeof(int) * (size_t)(nthreads + 8) +
+                       sizeof(float) * ((accum & KTM_INDEX_FAST) ? (size_t)p->row
+                                                                 : (size_t)(8 * K * KTM_WAVE));
     if (lds > 160 * 1024) return KTM_ERR_UNSUPPORTED;
     /* runs of a row of tiles, but at least 8 runs per XCD */
     a.run = a.ntiles / 64 < a.tiles_x ? a.ntiles / 64 : a.tiles_x;

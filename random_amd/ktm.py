@@ -17,7 +17,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import (ACCUM_KAHAN, ACCUM_PLAIN, PREP_REFERENCE, SEGY_IBM, TRACE_F32, KtmConfig,
+from ._lib import (ACCUM_KAHAN, ACCUM_PLAIN, INDEX_FAST, PREP_REFERENCE, SEGY_IBM, TRACE_F32, KtmConfig,
                    KtmImage, KtmSetup, KtmStats, KtmTraces, KtmVelocity, check, lib)
 
 
@@ -222,7 +222,8 @@ class Plan:
         """All traces in blocks of ``ntblock``: returns (km, fold, contributions)."""
         import torch
         block = block or self.cfg.ntblock
-        km, fold, comp = self.new_image(kahan=accum == ACCUM_KAHAN, device=traces.device)
+        km, fold, comp = self.new_image(kahan=(accum & ~INDEX_FAST) == ACCUM_KAHAN,
+                                        device=traces.device)
         counter = torch.zeros(1, dtype=torch.int64, device=traces.device)
         for i0 in range(0, traces.shape[0], block):
             prepared, geom = self.prep(traces[i0:i0 + block], coords[i0:i0 + block], mode, dtype)
