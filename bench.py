@@ -52,14 +52,15 @@ def parse():
     ap.add_argument("--ntr", type=int, default=1_000_000, help="survey traces")
     ap.add_argument("--block", type=int, default=50_000, help="traces per step (ntblock)")
     ap.add_argument("--ncols", type=int, default=400)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline time")
+    ap.add_argument("--cpu-stride", type=int, default=100,
+                    help="CPU baseline on every N-th trace (100: the 1 %% subset)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the ingest, velocity-gradient, adjoint/LSM, post-stack and EOM legs")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3")
     ap.add_argument("--profile-leg", default="fp32",
                     help="with --profile: the precision leg (a PRECISION_LEGS label) to run")
     ap.add_argument("--leg-steps", type=int, default=3, help="timed blocks per precision leg")
-    ap.add_argument("--err-traces", type=int, default=20000,
+    ap.add_argument("--err-traces", type=int, default=10000,
                     help="traces of the FP64-oracle error check on the 50x50x1500 sub-volume")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy 3x (HBM counter calibration) and exit")
@@ -118,7 +119,8 @@ VALU_MODES = {0: "v_fma_f32", 1: "v_add_f32", 2: "v_sqrt_f32", 3: "v_cvt_u32_f32
               "lshl_add, add)", 6: "1 v_sqrt_f32 : 4 v_fma_f32", 7: "1 v_sqrt_f32 : 1 v_fma_f32",
               8: "v_min_f32", 9: "v_rndne_f32", 10: "v_lshl_add_u32",
               11: "v_cmp_le_f32 (SGPR out; hazard nops: upper bound)", 12: "v_cndmask_b32",
-              13: "v_add_u32", 14: "v_fma_f64", 15: "v_readlane_b32 (hazard nops: upper bound)"}
+              13: "v_add_u32", 14: "v_fma_f64", 15: "v_readlane_b32 (hazard nops: upper bound)",
+              16: "v_rsq_f64", 17: "v_lshlrev_b64", 18: "v_mov_b32 (hazard nops: upper bound)"}
 
 
 def valu_probe(dev, modes=tuple(VALU_MODES), waves_per_simd=(8,), target_ms=40.0):
@@ -165,44 +167,108 @@ def valu_probe(dev, modes=tuple(VALU_MODES), waves_per_simd=(8,), target_ms=40.0
     return out
 
 
+# VALU instructions of one k_migrate launch on this workload (the first
+# 50,000-trace block) by class, from two rocprofv3 --pmc passes
+# (profiles/k_migrate_area3d_1e6.md).  Which instructions each counter
+# counts was read off ktm_probe_valu, whose instruction streams are known:
+# FMA_F32 = v_fma_f32/v_fmamk, ADD_F32 = v_add/v_sub_f32, TRANS_F32 =
+# v_sqrt_f32, CVT = conversions, INT32 = v_add_u32 and v_lshl_add_u32;
+# v_min/v_med3, v_rndne, v_cmp, v_cndmask, v_readlane are in no class.
+KMIG_COUNTS = {
+    "exact": {"kernel": "k_migrate<12, 2, 8, false, false>", "contributions": 1806379368000,
+              "launch_ms": 997.86, "valu": 514084083419, "fma_f32": 98855645105,
+              "add_f32": 96792681245, "mul_f32": 7096351279, "trans_f32": 57809899776,
+              "cvt": 34890724372, "int32": 56955079786, "fma_f64": 20902690694,
+              "add_f64": 1491403621, "mul_f64": 5980974484, "trans_f64": 2986647242,
+              "int64": 5715074464, "salu": 105936930242, "lds": 34905870923,
+              "clock_ghz": 2.383},
+}
+# the probe mode whose measured rate prices each class (SIMD cycles per
+# wave-instruction); INT32 at the v_add_u32 rate and the unclassified rest
+# at the v_min_f32 rate (v_med3/v_rndne/v_cmp/v_cndmask measure the same)
+CLASS_MODE = {"fma_f32": 0, "add_f32": 1, "mul_f32": 0, "trans_f32": 2, "cvt": 3, "int32": 13,
+              "fma_f64": 14, "add_f64": 14, "mul_f64": 14, "trans_f64": 16, "int64": 17,
+              "other": 8}
+
+
+def valu_roofline(counts, probe, contributions, launch_ms, ncu, clock_ghz):
+    """The VALU bound of one migration launch: the SIMD cycles its
+    instructions occupy at the probe's measured rates (instructions per class
+    scaled to this launch's contributions) over the SIMD cycles the launch
+    had (4 SIMDs per CU at the kernel's shader clock).  frac <= 1 whenever
+    the kernel issues no faster than the probe's independent streams."""
+    scale = contributions / counts["contributions"]
+    cyc = {c: 4.0 / probe[f"{m}@8"]["per_cu_cycle"] for c, m in CLASS_MODE.items()}
+    n = {c: counts[c] * scale for c in CLASS_MODE if c != "other"}
+    n["other"] = counts["valu"] * scale - sum(n.values())
+    busy = {c: n[c] * cyc[c] for c in n}
+    total = sum(busy.values())
+    secs = launch_ms * 1e-3
+    avail = 4.0 * ncu * clock_ghz * 1e9 * secs
+    f64 = sum(busy[c] for c in ("fma_f64", "add_f64", "mul_f64", "trans_f64"))
+    issue_peak = probe["0@8"]["wave_insts_per_s"]
+    return {"bound": "valu", "unit": "G SIMD-cycles/s", "achieved": round(total / secs / 1e9, 1),
+            "peak": round(4.0 * ncu * clock_ghz, 1), "frac": round(total / avail, 4),
+            "kernel": counts["kernel"], "avg_launch_ms": round(launch_ms, 2),
+            "shader_clock_ghz": clock_ghz,
+            "cycles_per_wave_instruction": {c: round(v, 2) for c, v in cyc.items()},
+            "valu_insts_per_contribution": round(counts["valu"] / counts["contributions"], 5),
+            "busy_share": {c: round(b / total, 4) for c, b in busy.items()},
+            "f64_exact_path_share": round(f64 / total, 4),
+            "issue_rate_frac_of_fma_peak": round(counts["valu"] * scale / secs / issue_peak, 4)}
+
+
 # ---- CPU-baseline leg: the only code in this file that runs the oracle
 # (oracle/), as the CPU timing beside the GPU figures and as the checker of
 # the GPU precision modes; every GPU figure comes from libktm.so. ----
 
-def cpu_baseline(sv, traces_np, target_s, nthreads):
-    """Oracle FP32 (the reference algorithm, strict IEEE, -O3) on the first S
-    traces over the whole image, jobs = one slab per thread as the reference's
-    MPI decomposition.  S is calibrated so the run takes about target_s.  The
-    same restatement built with the reference's own flags (-O0 -ffast-math,
-    Kirchhoff.c:3) is timed on a quarter of the sample beside it."""
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sv, traces, nthreads, stride=100, ref_stride=1000):
+    """Oracle FP32 (the reference algorithm restated, strict IEEE) on the
+    deterministic 1 % subset -- every stride-th trace of the survey -- over the
+    whole image, jobs = one image slab per thread (the reference's MPI
+    decomposition, Kirchhoff.c:280-295,347-350), built -O3 -march=native on
+    this host; the same restatement built with the reference's own flags
+    (-O0 -ffast-math, Kirchhoff.c:3) timed on every ref_stride-th trace.
+    Quotes the faster rate."""
+    import numpy as np
+
     import oracle
     cfg = sv.cfg.replace(jobs_x=nthreads, jobs_y=1)
-    n_cal = 48
     t0 = time.perf_counter()
-    _, _, nc = oracle.migrate(cfg, sv.ns, sv.dt, traces_np[:n_cal], sv.coords[:n_cal], sv.vel,
-                              nthreads=nthreads)
-    el = time.perf_counter() - t0
-    S = int(max(n_cal, min(traces_np.shape[0], n_cal * target_s / max(el, 1e-6))))
-    log(f"cpu baseline: calibration {nc:.3e} contrib in {el:.2f}s; sample {S} traces")
-    t0 = time.perf_counter()
-    _, _, nc = oracle.migrate(cfg, sv.ns, sv.dt, traces_np[:S], sv.coords[:S], sv.vel,
-                              nthreads=nthreads)
-    el = time.perf_counter() - t0
-    S4 = max(n_cal, S // 4)
-    t1 = time.perf_counter()
-    _, _, nc4 = oracle.migrate(cfg, sv.ns, sv.dt, traces_np[:S4], sv.coords[:S4], sv.vel,
-                               nthreads=nthreads, refflags=True)
-    el4 = time.perf_counter() - t1
-    log(f"cpu baseline: {nc:.3e} contrib in {el:.1f}s; reference flags {nc4:.3e} in {el4:.1f}s")
-    return {"value": round(nc / el / 1e9, 4), "unit": "G contributions/s", "cores": nthreads,
-            "kind": "port",
-            "sample": f"first {S} of {sv.ntr} traces over the whole {sv.cfg.end_iline}x"
-                      f"{sv.cfg.end_xline}x{sv.ns} image, {nc:.3e} contributions in {el:.1f} s on "
-                      f"{nthreads} threads, one job slab each (oracle/ktm_oracle.c, gcc -O3 "
-                      f"-march=x86-64-v3, strict IEEE)",
-            "reference_flags_value": round(nc4 / el4 / 1e9, 4),
-            "reference_flags_sample": f"first {S4} traces, same restatement built with -O0 "
-                                      f"-ffast-math (Kirchhoff.c:3), {el4:.1f} s"}
+    oracle.build_native()
+    log(f"cpu baseline: -march=native oracle built in {time.perf_counter() - t0:.1f}s")
+    runs = {}
+    for name, st, kw in (("O3_march_native", stride, {"native": True}),
+                         ("O0_ffast_math", ref_stride, {"refflags": True})):
+        idx = np.arange(0, sv.ntr, st)
+        tr = traces[idx].cpu().numpy()
+        co = np.ascontiguousarray(sv.coords[idx])
+        t0 = time.perf_counter()
+        _, _, nc = oracle.migrate(cfg, sv.ns, sv.dt, tr, co, sv.vel, nthreads=nthreads, **kw)
+        el = time.perf_counter() - t0
+        runs[name] = {"G_contributions_per_s": round(nc / el / 1e9, 4), "traces": int(idx.size),
+                      "trace_stride": st, "contributions": int(nc), "seconds": round(el, 1)}
+        log(f"cpu baseline {name}: {idx.size} traces, {nc:.3e} contributions in {el:.1f}s")
+    best = max(runs, key=lambda k: runs[k]["G_contributions_per_s"])
+    b = runs[best]
+    return {"value": b["G_contributions_per_s"], "unit": "G contributions/s", "cores": nthreads,
+            "kind": "port", "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "sample": f"every {b['trace_stride']}th trace of the {sv.ntr}-trace survey "
+                      f"({b['traces']} traces, {b['contributions']:.3e} contributions) over the "
+                      f"whole {sv.cfg.end_iline}x{sv.cfg.end_xline}x{sv.ns} image in "
+                      f"{b['seconds']} s on {nthreads} threads, one image slab each "
+                      f"(oracle/ktm_oracle.c, build {best})",
+            "builds": runs, "quoted_build": best}
 
 
 def cpu_baseline_subvolume(sv, cfg, tr_np, co_np):
@@ -232,14 +298,19 @@ def cpu_baseline_subvolume(sv, cfg, tr_np, co_np):
     return out
 
 
-def cpu_baseline_eom(cfg, sv, tr, coords):
+def cpu_baseline_eom(cfg, sv, tr, coords, nthreads):
     """CPU-baseline leg for equivalent-offset migration: the oracle's csp()
-    over the given traces, one thread.  Returns (samples, seconds)."""
+    over the given traces, its CSP grid split into one slab per thread (the
+    reference's decomposition, EOM.c), -O3 -march=native.  Returns
+    (samples, seconds)."""
     import numpy as np
 
     import oracle
+    oracle.build_native()
+    c = cfg.replace(jobs_x=nthreads, jobs_y=1)
     t0 = time.perf_counter()
-    _, _, ncpu = oracle.eom_all(cfg, sv.ns, np.float32(sv.dt), tr, coords, cfg.velocity(sv.ns))
+    _, _, ncpu = oracle.eom_all(c, sv.ns, np.float32(sv.dt), tr, coords, cfg.velocity(sv.ns),
+                                nthreads=nthreads, native=True)
     return ncpu, time.perf_counter() - t0
 
 # ---- end of the CPU-baseline leg ----
@@ -605,7 +676,7 @@ def host_leg(sv, traces, ntr=100_000):
             "host_bytes_out": img_bytes}
 
 
-def eom_leg(dev, reps=3, cpu_traces=48):
+def eom_leg(dev, reps=3, cpu_stride=10):
     """Equivalent-offset migration (EOM.c csp()) of the 40-shot line:
     4800 traces x 1001 samples into 800 CSPs x 160 offset bins, plus the post
     filter; the CPU oracle timed on the first traces of the same survey."""
@@ -635,14 +706,17 @@ def eom_leg(dev, reps=3, cpu_traces=48):
         ms_p.append(e1.elapsed_time(e2))
     n = int(counter.item())
     ms_b, ms_p = float(np.median(ms_b)), float(np.median(ms_p))
-    ncpu, el = cpu_baseline_eom(cfg, sv, tr[:cpu_traces], sv.coords[:cpu_traces])
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
+    ncpu, el = cpu_baseline_eom(cfg, sv, tr[::cpu_stride], sv.coords[::cpu_stride], nthreads)
     log(f"eom: binning {ms_b:.2f} ms, post {ms_p:.2f} ms, {n:.3e} samples; cpu {ncpu:.3e} in {el:.2f}s")
     return {"workload": "40-shot line, 4800 traces x 1001 samples -> 800 CSPs x 160 "
                         "offset bins of 12.5 m, aperture 2000 m, v = 2000 m/s",
             "evaluated_samples": n, "ms_binning": round(ms_b, 2), "ms_post": round(ms_p, 2),
             "G_samples_per_s": round(n / (ms_b * 1e-3) / 1e9, 2),
             "cpu_oracle_G_samples_per_s": round(ncpu / el / 1e9, 4),
-            "cpu_sample": f"first {cpu_traces} traces, one thread, {el:.2f} s"}
+            "cpu_sample": f"every {cpu_stride}th trace ({tr[::cpu_stride].shape[0]} traces), "
+                          f"{nthreads} threads (one CSP slab each), -O3 -march=native, "
+                          f"{el:.2f} s"}
 
 
 def main():
@@ -749,21 +823,31 @@ def main():
                    "traces": sv.ntr, "ns": sv.ns, "image": list(plan.image_shape[:2]) + [sv.ns],
                    "block_traces": args.block, "contributions": n_contrib,
                    "effective_contributions": run_mode.effective},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC,
-                     "kernel": "k_migrate", "bytes_per_contribution": BYTES_FP32,
-                     "avg_launch_ms": round(float(np.mean(mig_ms)), 2)},
     }
-    if not args.profile:
+    avg_ms = float(np.mean(mig_ms))
+    if args.profile:
+        result["roofline"] = {"kernel": KMIG_COUNTS["exact"]["kernel"], "avg_launch_ms": avg_ms}
+    else:
+        ncu = torch.cuda.get_device_properties(dev).multi_processor_count
+        probe = valu_probe(dev, modes=sorted(set(CLASS_MODE.values())))
+        result["roofline"] = valu_roofline(KMIG_COUNTS["exact"], probe, per_launch, avg_ms, ncu,
+                                           KMIG_COUNTS["exact"]["clock_ghz"])
         copy_gbs, lds_gbs = measured_peaks(dev)
-        traffic_gbs = TRAFFIC / (np.mean(mig_ms) * 1e-3) / 1e9
+        traffic_gbs = TRAFFIC / (avg_ms * 1e-3) / 1e9
         # the kernel reads one 4-byte word of LDS per contribution (the three
         # taps pre-combined into F(T) when the trace is staged)
-        lds_read = per_launch * 4 / (np.mean(mig_ms) * 1e-3) / 1e9
+        lds_read = per_launch * 4 / (avg_ms * 1e-3) / 1e9
         result["roofline"].update({
-            "hbm_copy_peak_measured": round(copy_gbs, 1),
+            "traffic": TRAFFIC,
             "hbm_traffic_GB_per_s": round(traffic_gbs, 1),
+            "hbm_traffic_frac": round(traffic_gbs / HBM_PEAK_GBS, 4),
+            "hbm_copy_peak_measured": round(copy_gbs, 1),
             "hbm_traffic_frac_of_copy_peak": round(traffic_gbs / copy_gbs, 4),
+            "algorithmic_bytes_rate": {
+                "bytes_per_contribution": BYTES_FP32, "GB_per_s": round(achieved, 1),
+                "note": "the three FP32 taps of Kirchhoff.c:1392 per contribution; the kernel "
+                        "reads one pre-combined tap per contribution from LDS, so this rate "
+                        "is not HBM traffic and exceeds the HBM peak"},
             "lds": {"read_bytes_per_contribution": 4, "achieved": round(lds_read, 1),
                     "peak_measured": round(lds_gbs, 1), "frac": round(lds_read / lds_gbs, 4),
                     "unit": "GB/s"}})
@@ -798,9 +882,7 @@ def main():
     if not args.no_cpu and not args.profile:
         nthreads = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
         nthreads = max(1, min(nthreads, 64))
-        sample = max(64, min(sv.ntr, 4000))
-        tr_np = traces[:sample].cpu().numpy()
-        result["cpu_baseline"] = cpu_baseline(sv, tr_np, args.cpu_seconds, nthreads)
+        result["cpu_baseline"] = cpu_baseline(sv, traces, nthreads, stride=args.cpu_stride)
         result["cpu_baseline"]["speedup_gpu_over_cpu"] = round(gcs / result["cpu_baseline"]["value"], 1)
     print(json.dumps(result), flush=True)
 

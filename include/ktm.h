@@ -482,7 +482,10 @@ int ktm_probe_lds(uint32_t *out, int nblocks, int iters, void *stream);
  * inline-assembly VALU sequence over 8 independent register chains: mode 0
  * v_fma_f32, 1 v_add_f32, 2 v_sqrt_f32, 3 v_cvt_u32_f32/v_cvt_f32_u32,
  * 4 one dependent v_fma_f32 chain, 5 the migration tap (2 fma, 2 sqrt, add,
- * min, cvt, lshl_add, add), 6 one sqrt per four fma, 7 one sqrt per fma.
+ * min, cvt, lshl_add, add), 6 one sqrt per four fma, 7 one sqrt per fma,
+ * 8 v_min_f32, 9 v_rndne_f32, 10 v_lshl_add_u32, 11 v_cmp_le_f32, 12
+ * v_cndmask_b32, 13 v_add_u32, 14 v_fma_f64, 15 v_readlane_b32, 16 v_rsq_f64,
+ * 17 v_lshlrev_b64, 18 v_mov_b32.
  * ktm_probe_valu_ops(mode) is the VALU instruction count per lane and
  * iteration (-1 for a bad mode).  out (device) receives 3 words per
  * workgroup: shader-clock cycles (clock64) and 100 MHz ticks (wall_clock64)

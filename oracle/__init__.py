@@ -17,6 +17,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "build", "libktm_oracle.so")
 LIB_REFFLAGS_PATH = os.path.join(_HERE, "build", "libktm_oracle_refflags.so")
+LIB_NATIVE_PATH = os.path.join(_HERE, "build", "libktm_oracle_native.so")
 
 FP32, FP64_DATA, FP64_FULL = 0, 1, 2
 
@@ -73,8 +74,16 @@ _PROTOS = {
 _libs: dict = {}
 
 
-def lib(refflags: bool = False) -> C.CDLL:
-    path = LIB_REFFLAGS_PATH if refflags else LIB_PATH
+def build_native() -> str:
+    """Compile the -O3 -march=native build for the CPU this process runs on
+    (make -C oracle native); returns its path."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE, "native"], check=True, capture_output=True)
+    return LIB_NATIVE_PATH
+
+
+def lib(refflags: bool = False, native: bool = False) -> C.CDLL:
+    path = LIB_REFFLAGS_PATH if refflags else (LIB_NATIVE_PATH if native else LIB_PATH)
     if path not in _libs:
         if not os.path.exists(path):
             raise RuntimeError(f"{path} missing: run `make -C oracle`")
@@ -156,7 +165,7 @@ def trace_geometry(cfg, coords: np.ndarray):
 
 
 def migrate(cfg, ns, dt, traces, coords, vel, mode=FP32, nthreads=None, dtm=None,
-            refflags=False):
+            refflags=False, native=False):
     """All jobs into the padded image [nx*jobs_x][ny*jobs_y][nh][ns].
     Returns (km, fold, contributions)."""
     p, g = params(cfg), grid(cfg)
@@ -168,7 +177,7 @@ def migrate(cfg, ns, dt, traces, coords, vel, mode=FP32, nthreads=None, dtm=None
     km = np.zeros(shape, dtype)
     fold = np.zeros(shape, dtype)
     nthreads = nthreads or os.cpu_count() or 1
-    n = lib(refflags).orc_migrate_all(C.byref(p), ns, dt, vel.shape[2], dt if dtm is None else dtm,
+    n = lib(refflags, native).orc_migrate_all(C.byref(p), ns, dt, vel.shape[2], dt if dtm is None else dtm,
                                       _vpn(traces), _vpn(coords), traces.shape[0], _vpn(vel), mode,
                                       _vpn(km), _vpn(fold), nthreads)
     if n < 0:
@@ -339,7 +348,7 @@ def eom_filters(cfg, ns: int, dt: float):
     return lp, rr
 
 
-def eom_job(cfg, ns, dt, job, traces, coords, vel):
+def eom_job(cfg, ns, dt, job, traces, coords, vel, native=False):
     """(csp, fold, evaluated) of one job slab [nx][ny][nh][ns]."""
     p, g = eom_params(cfg), eom_grid(cfg)
     traces = np.ascontiguousarray(traces, np.float32)
@@ -348,21 +357,26 @@ def eom_job(cfg, ns, dt, job, traces, coords, vel):
     shape = (g.nx, g.ny, cfg.nh, ns)
     csp = np.zeros(shape, np.float32)
     fold = np.zeros(shape, np.float32)
-    n = lib().orc_eom_job(C.byref(p), ns, dt, job, _vpn(traces), _vpn(coords), traces.shape[0],
-                          _vpn(vel), _vpn(csp), _vpn(fold))
+    n = lib(native=native).orc_eom_job(C.byref(p), ns, dt, job, _vpn(traces), _vpn(coords),
+                                       traces.shape[0], _vpn(vel), _vpn(csp), _vpn(fold))
     if n < 0:
         raise ValueError("oracle: bad job")
     return csp, fold, int(n)
 
 
-def eom_all(cfg, ns, dt, traces, coords, vel):
-    """All jobs into the padded gather volume [nx*jobs_x][ny*jobs_y][nh][ns]."""
+def eom_all(cfg, ns, dt, traces, coords, vel, nthreads=1, native=False):
+    """All jobs into the padded gather volume [nx*jobs_x][ny*jobs_y][nh][ns],
+    the jobs on nthreads threads (the reference's one rank per slab; ctypes
+    releases the GIL during each job)."""
+    from concurrent.futures import ThreadPoolExecutor
     g = eom_grid(cfg)
     out_c = np.zeros((g.nx * cfg.jobs_x, g.ny * cfg.jobs_y, cfg.nh, ns), np.float32)
     out_f = np.zeros_like(out_c)
+    with ThreadPoolExecutor(max_workers=max(1, nthreads)) as ex:
+        res = list(ex.map(lambda job: eom_job(cfg, ns, dt, job, traces, coords, vel, native),
+                          range(g.no_job)))
     total = 0
-    for job in range(g.no_job):
-        c, f, n = eom_job(cfg, ns, dt, job, traces, coords, vel)
+    for job, (c, f, n) in enumerate(res):
         xs, ys = job // cfg.jobs_y, job % cfg.jobs_y
         out_c[xs * g.nx:(xs + 1) * g.nx, ys * g.ny:(ys + 1) * g.ny] = c
         out_f[xs * g.nx:(xs + 1) * g.nx, ys * g.ny:(ys + 1) * g.ny] = f

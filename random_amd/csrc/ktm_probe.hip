@@ -32,6 +32,11 @@
  *  13 v_add_u32                                                 [32]
  *  14 v_fma_f64                                                 [32]
  *  15 v_readlane_b32                                            [32]
+ *  16 v_rsq_f64 (transcendental)                                [32]
+ *  17 v_lshlrev_b64                                             [32]
+ *  18 v_mov_b32                                                 [32]
+ * For modes 11, 15 and 18 the compiler inserts hazard waits between the
+ * inline-assembly instructions: their rates are lower bounds.
  */
 #include <hip/hip_runtime.h>
 
@@ -176,6 +181,18 @@ __global__ void __launch_bounds__(PV_NT) k_probe_valu(uint32_t *__restrict__ out
 #define OP(k) asm volatile("v_readlane_b32 %0, %1, %2" : "=s"(sr[k]) : "v"(x[k]), "s"(lanesel))
             for (int u = 0; u < 4; ++u) ALL8(OP);
 #undef OP
+        } else if (MODE == 16) {
+#define OP(k) asm volatile("v_rsq_f64 %0, %0" : "+v"(z[k]))
+            for (int u = 0; u < 4; ++u) ALL8(OP);
+#undef OP
+        } else if (MODE == 17) {
+#define OP(k) asm volatile("v_lshlrev_b64 %0, 1, %0" : "+v"(z[k]))
+            for (int u = 0; u < 4; ++u) ALL8(OP);
+#undef OP
+        } else if (MODE == 18) {
+#define OP(k) asm volatile("v_mov_b32 %0, %1" : "=v"(w[k]) : "v"(x[k]))
+            for (int u = 0; u < 4; ++u) ALL8(OP);
+#undef OP
         }
     }
     const uint64_t c1 = clock64();
@@ -202,20 +219,22 @@ __global__ void __launch_bounds__(PV_NT) k_probe_valu(uint32_t *__restrict__ out
  * assembly above) */
 extern "C" int ktm_probe_valu_ops(int mode)
 {
-    static const int ops[16] = {32, 32, 32, 32, 32, 72, 40, 32, 32, 32, 32, 32, 32, 32, 32, 32};
-    return mode >= 0 && mode < 16 ? ops[mode] : -1;
+    static const int ops[19] = {32, 32, 32, 32, 32, 72, 40, 32, 32, 32,
+                                32, 32, 32, 32, 32, 32, 32, 32, 32};
+    return mode >= 0 && mode < 19 ? ops[mode] : -1;
 }
 
 /* out (device): 3 words per workgroup -- shader-clock cycles and 100 MHz
  * ticks of its loop, and a checksum */
 extern "C" int ktm_probe_valu(int mode, uint32_t *out, int nblocks, int iters, void *stream)
 {
-    if (!out || nblocks < 1 || iters < 1 || mode < 0 || mode > 15) return KTM_ERR_ARG;
+    if (!out || nblocks < 1 || iters < 1 || mode < 0 || mode > 18) return KTM_ERR_ARG;
     typedef void (*fn_t)(uint32_t *, int, float);
-    static const fn_t fns[16] = {k_probe_valu<0>, k_probe_valu<1>, k_probe_valu<2>, k_probe_valu<3>,
+    static const fn_t fns[19] = {k_probe_valu<0>, k_probe_valu<1>, k_probe_valu<2>, k_probe_valu<3>,
                                  k_probe_valu<4>, k_probe_valu<5>, k_probe_valu<6>, k_probe_valu<7>,
                                  k_probe_valu<8>, k_probe_valu<9>, k_probe_valu<10>, k_probe_valu<11>,
-                                 k_probe_valu<12>, k_probe_valu<13>, k_probe_valu<14>, k_probe_valu<15>};
+                                 k_probe_valu<12>, k_probe_valu<13>, k_probe_valu<14>, k_probe_valu<15>,
+                                 k_probe_valu<16>, k_probe_valu<17>, k_probe_valu<18>};
     hipLaunchKernelGGL(fns[mode], dim3((unsigned)nblocks), dim3(PV_NT), 0, (hipStream_t)stream,
                        out, iters, 1.5f);
     return hipGetLastError() == hipSuccess ? KTM_OK : KTM_ERR_DEVICE;

@@ -7,8 +7,8 @@ import pytest
 import torch
 
 import oracle
-from random_amd import (ACCUM_KAHAN, ACCUM_PLAIN, PREP_CLOSED, PREP_REFERENCE, TRACE_BF16,
-                        TRACE_F16, TRACE_F32, surveys)
+from random_amd import (ACCUM_KAHAN, ACCUM_PLAIN, INDEX_FAST, PREP_CLOSED, PREP_REFERENCE,
+                        TRACE_BF16, TRACE_F16, TRACE_F32, _lib, surveys)
 from random_amd.ktm import Plan
 
 pytestmark = pytest.mark.gpu
@@ -65,6 +65,93 @@ def test_migration_error_vs_fp64(cuda, area, name, mode, dtype, accum, tol):
     assert n == n64
     assert np.array_equal(fold.cpu().numpy(), fold64.astype(np.float32))
     assert r <= tol
+
+
+@pytest.fixture(scope="module")
+def area_full64(area):
+    """The same survey through the oracle with every float widened
+    (FP64_FULL): an FP64 traveltime index as well as FP64 data."""
+    sv, tr, *_ = area
+    km, fold, n = oracle.migrate(sv.cfg, sv.ns, sv.dt, tr, sv.coords, sv.vel,
+                                 mode=oracle.FP64_FULL)
+    return km, fold, n
+
+
+@pytest.mark.parametrize("name,dtype,accum,tol", [
+    ("fp32-closed-fast", TRACE_F32, ACCUM_PLAIN | INDEX_FAST, 5e-3),
+    ("fp32-closed-kahan-fast", TRACE_F32, ACCUM_KAHAN | INDEX_FAST, 5e-3),
+    ("bf16-kahan-fast", TRACE_BF16, ACCUM_KAHAN | INDEX_FAST, 5e-3),
+])
+def test_fast_index_error_vs_fp64(cuda, area, area_full64, name, dtype, accum, tol):
+    """KTM_INDEX_FAST: floor of the single-precision fast form instead of the
+    reference's index.  Against the FP64 oracle with the reference's index
+    the error is the index deviation (tol); against an FP64 traveltime
+    (FP64_FULL) the fast index is within 25 % of the reference index's own
+    error, i.e. it is as good an FP32 approximation of the traveltime; the
+    contribution count is the reference's and fold differs only where a
+    sample's index crosses the end of the trace."""
+    sv, tr, km64, fold64, n64 = area
+    kmf, _, _ = area_full64
+    print(f"case: {name} vs FP64 oracles, 3000 traces, 32x32x1500", flush=True)
+    plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
+    d_tr, d_co = torch.from_numpy(tr).to(cuda), torch.from_numpy(sv.coords).to(cuda)
+    km, fold, n = plan.migrate(d_tr, d_co, mode=PREP_CLOSED, dtype=dtype, accum=accum,
+                               block=1000)
+    ke, _, _ = plan.migrate(d_tr, d_co, mode=PREP_CLOSED, dtype=dtype,
+                            accum=accum & ~INDEX_FAST, block=1000)
+    km, ke = km.cpu().numpy(), ke.cpu().numpy()
+    r = rel_l2(km, km64)
+    r_full, r_exact_full = rel_l2(km, kmf), rel_l2(ke, kmf)
+    df = np.abs(fold.cpu().numpy() - fold64)
+    print(f"  relL2 vs FP64 (reference index) {r:.3e}; vs FP64 traveltime: fast {r_full:.3e}, "
+          f"reference index {r_exact_full:.3e}; fold differs at {np.mean(df > 0):.2e}", flush=True)
+    assert n == n64
+    assert r <= tol
+    assert r_full <= 1.25 * r_exact_full
+    assert np.mean(df > 0) < 1e-3 and df.max() <= 4
+
+
+def test_fast_index_mismatch_rate(cuda, area):
+    """How often floor(q) of the fast form differs from the reference's
+    index, on (trace, column, sample) triples of the survey within the
+    aperture: below 1e-4 (measured ~3e-5 on the 3-D workload), and the
+    kernel's exact decision procedure never differs."""
+    sv, *_ = area
+    rng = np.random.default_rng(11)
+    plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
+    geo, _ = oracle.trace_geometry(sv.cfg, sv.coords)
+    n = 1 << 21
+    i = rng.integers(0, sv.ntr, n)
+    X = rng.integers(0, 32, n)
+    Y = rng.integers(0, 32, n)
+    t = rng.integers(0, sv.ns, n)
+    IX = (X * sv.cfg.iline_distance).astype(np.float32)
+    IY = (Y * sv.cfg.xline_distance).astype(np.float32)
+    g = geo[i]
+    ok = (IX - g[:, 4]) ** 2 + (IY - g[:, 5]) ** 2 < np.float32(sv.cfg.aperture) ** 2
+    dis2 = (IX - g[:, 0]) ** 2 + (IY - g[:, 1]) ** 2
+    dir2 = (IX - g[:, 2]) ** 2 + (IY - g[:, 3]) ** 2
+    v2 = sv.vel[X, Y, t] * sv.vel[X, Y, t]
+    inp = np.ascontiguousarray(np.stack([dis2, dir2, t.astype(np.float32), v2], 1)[ok],
+                               np.float32)
+    m = inp.shape[0]
+    d_in = torch.from_numpy(inp).to(cuda)
+    qf = torch.empty(m, dtype=torch.float32, device=cuda)
+    qr = torch.empty_like(qf)
+    tk = torch.empty(m, dtype=torch.int32, device=cuda)
+    trf = torch.empty_like(tk)
+    _lib.check(_lib.lib().ktm_selftest_traveltime(d_in.data_ptr(), m, float(sv.dt), sv.ns,
+                                                  qf.data_ptr(), qr.data_ptr(), tk.data_ptr(),
+                                                  trf.data_ptr(), None), "selftest_traveltime")
+    torch.cuda.synchronize()
+    q = qf.cpu().numpy()
+    t_fast = np.where(q < sv.ns, np.floor(q), sv.ns).astype(np.int64)
+    t_ref = trf.cpu().numpy()
+    rate = float(np.mean(t_fast != t_ref))
+    print(f"case: fast-form index vs reference on {m} samples: mismatch rate {rate:.2e}",
+          flush=True)
+    assert 0 < rate < 1e-4
+    assert np.array_equal(tk.cpu().numpy(), t_ref)
 
 
 def test_reference_fp32_error_is_the_references_own(cuda, area):
