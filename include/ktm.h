@@ -202,6 +202,23 @@ int ktm_migrate_block(const ktm_plan *plan, const void *prepared, int trace_dtyp
 /* job < 0 migrates every slab; job >= 0 only the columns of that job's slab
  * (Kirchhoff.c:1341-1347), leaving the rest of the image untouched. */
 
+/* ktm_migrate_block without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  Same
+ * arguments and results as ktm_migrate_block (bit for bit); the BF16/FP16
+ * decode workspace, which ktm_migrate_block allocates in the plan on first
+ * use, is instead a fixed device array of KTM_STATIC_STAGE_FLOATS floats
+ * (64 MiB) that each GPU holds until the process ends, so a BF16/FP16 call
+ * takes at most KTM_STATIC_STAGE_FLOATS / ktm_plan_trace_row(plan) traces
+ * (KTM_ERR_UNSUPPORTED beyond; FP32 storage needs no workspace).  Every call
+ * on one GPU shares that array: calls that overlap on a GPU (other streams,
+ * other plans, other threads) race and must be serialised by the caller. */
+#define KTM_STATIC_STAGE_FLOATS (16u << 20)
+int ktm_migrate_block_static(const ktm_plan *plan, const void *prepared, int trace_dtype,
+                             int prep_mode, const ktm_trace_geom *geom, int64_t ntr,
+                             int accum, int job, float *km, float *fold, float *comp,
+                             uint64_t *counter, void *stream);
+
 /* Geometry only (device): coords[ntr][4] -> geom[ntr], Kirchhoff.c:1296-1310. */
 int ktm_geometry(const ktm_plan *plan, const int32_t *coords, int64_t ntr,
                  ktm_trace_geom *geom, void *stream);

@@ -123,6 +123,7 @@ _PROTOS = {
     "ktm_prep": (_i32, [_vp, _vp, _vp, _i64, _i32, _i32, _vp, _vp, _vp]),
     "ktm_image_reset": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "ktm_migrate_block": (_i32, [_vp, _vp, _i32, _i32, _vp, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp]),
+    "ktm_migrate_block_static": (_i32, [_vp, _vp, _i32, _i32, _vp, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp]),
     "ktm_poststack": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "ktm_migrate": (_i32, [C.POINTER(KtmConfig), C.POINTER(KtmTraces), C.POINTER(KtmVelocity),
                            C.POINTER(KtmImage), C.POINTER(KtmStats)]),

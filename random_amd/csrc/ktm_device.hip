@@ -33,6 +33,8 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <mutex>
+
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -70,7 +72,10 @@ struct ktm_plan {
     void *d_dm_tmp;
     size_t dm_cap, dm_tmp_bytes;
     uint32_t *d_dm_amax;    /* bits of max |km| of the last demigration */
+    float *d_scratch;       /* this device's copy of `scratch` (ktm_migrate_block_static) */
 };
+
+static int scratch_address(int dev, float **out);
 
 #define HIP_TRY(expr)                                  \
     do {                                               \
@@ -1441,6 +1446,10 @@ extern "C" int ktm_plan_create(ktm_plan **out, const ktm_config *cfg, int ns, fl
     int st = ktm_tables_build(cfg, ns, dt, nsm, dtm, &p->tb);
     if (st != KTM_OK) { free(p); return st; }
     if (hipGetDevice(&p->device) != hipSuccess) { ktm_plan_destroy(p); return KTM_ERR_DEVICE; }
+    if ((st = scratch_address(p->device, &p->d_scratch)) != KTM_OK) {
+        ktm_plan_destroy(p);
+        return st;
+    }
     p->row = ((int64_t)ns + 1 + 7) / 8 * 8;
     /* square the velocity model (Kirchhoff.c:617) and check the part the
      * image uses: a velocity <= 0 (or NaN) would divide by zero at 1381 */
@@ -1891,19 +1900,26 @@ static mig_fn pick_migrate(int ns, int accum, int64_t wgs8, int *W, int *K, int 
     return pick_accum<12, 8, 5>(accum, B);
 }
 
-/* FP32 filtered traces of a BF16/FP16 block, in a workspace the plan owns and
- * grows on demand (the only allocation on the migration path) */
+/* FP32 filtered traces of a BF16/FP16 block, into stage (ntr * row floats)
+ * or, when stage is NULL, into a workspace the plan owns and grows on demand
+ * (the only allocation on the migration path) */
 static int decode_block(const ktm_plan *cp, const void *prepared, int dtype,
-                        const ktm_trace_geom *geom, int64_t ntr, hipStream_t s, const float **out)
+                        const ktm_trace_geom *geom, int64_t ntr, hipStream_t s, float *stage,
+                        const float **out)
 {
     ktm_plan *p = const_cast<ktm_plan *>(cp);
-    const size_t need = (size_t)ntr * (size_t)p->row;
-    if (need > p->stage_cap) {
-        if (p->d_stage) HIP_TRY(hipFree(p->d_stage));
-        p->d_stage = nullptr;
-        p->stage_cap = 0;
-        if (hipMalloc((void **)&p->d_stage, need * sizeof(float)) != hipSuccess) return KTM_ERR_NOMEM;
-        p->stage_cap = need;
+    float *dst = stage;
+    if (!dst) {
+        const size_t need = (size_t)ntr * (size_t)p->row;
+        if (need > p->stage_cap) {
+            if (p->d_stage) HIP_TRY(hipFree(p->d_stage));
+            p->d_stage = nullptr;
+            p->stage_cap = 0;
+            if (hipMalloc((void **)&p->d_stage, need * sizeof(float)) != hipSuccess)
+                return KTM_ERR_NOMEM;
+            p->stage_cap = need;
+        }
+        dst = p->d_stage;
     }
     const int nt = 256;
     for (int64_t i0 = 0; i0 < ntr; i0 += 65535) {
@@ -1912,20 +1928,74 @@ static int decode_block(const ktm_plan *cp, const void *prepared, int dtype,
         const TraceSrc src = {reinterpret_cast<const uint16_t *>(prepared) + i0 * p->row, p->row};
         if (dtype == KTM_TRACE_BF16)
             hipLaunchKernelGGL(k_prep_closed<SRC_Q_BF16>, grid, dim3(nt), 0, s, src, geom + i0, n,
-                               p->tb.s.ns, p->row, p->d_w, p->d_stage + i0 * p->row);
+                               p->tb.s.ns, p->row, p->d_w, dst + i0 * p->row);
         else
             hipLaunchKernelGGL(k_prep_closed<SRC_Q_F16>, grid, dim3(nt), 0, s, src, geom + i0, n,
-                               p->tb.s.ns, p->row, p->d_w, p->d_stage + i0 * p->row);
+                               p->tb.s.ns, p->row, p->d_w, dst + i0 * p->row);
     }
     HIP_TRY(hipGetLastError());
-    *out = p->d_stage;
+    *out = dst;
     return KTM_OK;
 }
+
+static int migrate_block(const ktm_plan *p, const void *prepared, int dtype, int prep_mode,
+                         const ktm_trace_geom *geom, int64_t ntr, int accum, int job, float *km,
+                         float *fold, float *comp, uint64_t *counter, void *stream, float *stage);
 
 extern "C" int ktm_migrate_block(const ktm_plan *p, const void *prepared, int dtype, int prep_mode,
                                  const ktm_trace_geom *geom, int64_t ntr, int accum, int job,
                                  float *km, float *fold, float *comp, uint64_t *counter,
                                  void *stream)
+{
+    return migrate_block(p, prepared, dtype, prep_mode, geom, ntr, accum, job, km, fold, comp,
+                         counter, stream, nullptr);
+}
+
+/* ---- the allocation-free entry point ----
+ * The BF16/FP16 decode workspace of ktm_migrate_block_static is this array:
+ * every GPU the library runs on holds its own copy (KTM_STATIC_STAGE_FLOATS
+ * floats, 64 MiB) until the process ends.  Host code never passes the name
+ * `scratch` to a kernel (on the host it is only a stand-in for the symbol):
+ * each plan keeps the device address hipGetSymbolAddress gave for its device
+ * at ktm_plan_create, looked up once per device. */
+static __device__ __align__(256) float scratch[KTM_STATIC_STAGE_FLOATS];
+
+#define KTM_MAX_DEVICES 64
+static float *g_scratch_ptr[KTM_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+/* the device address of `scratch` on device dev, which must be current */
+static int scratch_address(int dev, float **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (dev < 0 || dev >= KTM_MAX_DEVICES) return KTM_ERR_UNSUPPORTED;
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr)
+            return KTM_ERR_DEVICE;
+        g_scratch_ptr[dev] = reinterpret_cast<float *>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return KTM_OK;
+}
+
+extern "C" int ktm_migrate_block_static(const ktm_plan *p, const void *prepared, int dtype,
+                                        int prep_mode, const ktm_trace_geom *geom, int64_t ntr,
+                                        int accum, int job, float *km, float *fold, float *comp,
+                                        uint64_t *counter, void *stream)
+{
+    if (!p || ntr < 0) return KTM_ERR_ARG;
+    if (!p->d_scratch) return KTM_ERR_DEVICE;
+    if (dtype != KTM_TRACE_F32 &&
+        (uint64_t)ntr * (uint64_t)p->row > (uint64_t)KTM_STATIC_STAGE_FLOATS)
+        return KTM_ERR_UNSUPPORTED;
+    return migrate_block(p, prepared, dtype, prep_mode, geom, ntr, accum, job, km, fold, comp,
+                         counter, stream, p->d_scratch);
+}
+
+static int migrate_block(const ktm_plan *p, const void *prepared, int dtype, int prep_mode,
+                         const ktm_trace_geom *geom, int64_t ntr, int accum, int job, float *km,
+                         float *fold, float *comp, uint64_t *counter, void *stream, float *stage)
 {
     if (!p || !km || !fold || ntr < 0) return KTM_ERR_ARG;
     if (!ntr) return KTM_OK;
@@ -1943,7 +2013,7 @@ extern "C" int ktm_migrate_block(const ktm_plan *p, const void *prepared, int dt
     hipStream_t hs = (hipStream_t)stream;
     const float *F = reinterpret_cast<const float *>(prepared);
     if (dtype != KTM_TRACE_F32) {
-        const int st = decode_block(p, prepared, dtype, geom, ntr, hs, &F);
+        const int st = decode_block(p, prepared, dtype, geom, ntr, hs, stage, &F);
         if (st != KTM_OK) return st;
     }
     MigArgs a;
