@@ -10,7 +10,12 @@
  *
  * Pointers documented as "device" are HIP device memory; `stream` is a
  * hipStream_t passed as void* (NULL = the null stream).  A plan is not
- * thread-safe; distinct plans may be used from distinct threads.
+ * thread-safe; distinct plans may be used from distinct threads.  A plan's
+ * workspaces (the BF16/FP16 decode buffer of ktm_migrate_block, the
+ * demigration sort keys and max-|km| word of ktm_demigrate) are reused by
+ * every call without stream ordering: all calls on one plan must be issued
+ * on a single stream (or the caller orders them), or a later call can
+ * overwrite a workspace an earlier call's kernels still read.
  */
 #ifndef KTM_H
 #define KTM_H

@@ -31,7 +31,7 @@
  * divide/sqrt; packed-FP32 ops are disabled.
  */
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <mutex>
 
@@ -2148,8 +2148,8 @@ static int demig_order(const ktm_plan *cp, const ktm_trace_geom *geom, int64_t n
     uint64_t *k_in = p->d_dm_keys, *k_out = p->d_dm_keys + n;
     int32_t *i_in = p->d_dm_idx, *i_out = p->d_dm_idx + n;
     size_t need = 0;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, need, k_in, k_out, i_in, i_out, (int)n, 0,
-                                               64, hs));
+    HIP_TRY(rocprim::radix_sort_pairs(nullptr, need, k_in, k_out, i_in, i_out, (size_t)n, 0u, 64u,
+                                      hs));
     if (need > p->dm_tmp_bytes) {
         if (p->d_dm_tmp) HIP_TRY(hipFree(p->d_dm_tmp));
         p->d_dm_tmp = nullptr;
@@ -2161,8 +2161,8 @@ static int demig_order(const ktm_plan *cp, const ktm_trace_geom *geom, int64_t n
                        s->nh, p->d_colx, s->nxp, p->d_coly, s->nyp, k_in, i_in);
     HIP_TRY(hipGetLastError());
     size_t have = p->dm_tmp_bytes;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(p->d_dm_tmp, have, k_in, k_out, i_in, i_out, (int)n,
-                                               0, 64, hs));
+    HIP_TRY(rocprim::radix_sort_pairs(p->d_dm_tmp, have, k_in, k_out, i_in, i_out, (size_t)n, 0u,
+                                      64u, hs));
     *order = i_out;
     return KTM_OK;
 }

@@ -86,7 +86,11 @@ def _stream():
 
 
 class Plan:
-    """Device-resident setup of one configuration (``ktm_plan``)."""
+    """Device-resident setup of one configuration (``ktm_plan``).
+
+    Every call is issued on torch's current stream; the plan's workspaces
+    (16-bit decode buffer, demigration sort keys) are reused without stream
+    ordering, so all calls on one plan must run on a single stream."""
 
     def __init__(self, cfg: Config, ns: int, dt: float, vel: np.ndarray, dtm: float | None = None):
         import torch  # noqa: F401  (device context comes from torch)
