@@ -613,29 +613,61 @@ __device__ __forceinline__ float clamp_ns(float q, float hi)
     return __builtin_amdgcn_fmed3f(q, 0.0f, hi);
 }
 
+/* Queue of the speculative exact path: items per lane and staged batch (a
+ * lane's further ambiguous samples of the batch take the inline exact path) */
+#define KTM_SPEC_Q 1
+
+/* The reference traveltime index of a sample with every rule of the exact
+ * path: the fast form, and where an integer lies within its error bound the
+ * reference arithmetic, ns when the sample is past the trace for certain. */
+__device__ __forceinline__ int reference_T(float dIS2, float dIR2, float rs, float tq, float t2,
+                                           float v2, float dt, float rdt, int ns)
+{
+    constexpr float EPS = KTM_FAST_EPS_S;
+    const float nsf = (float)ns;
+    const float q = fast_qs(dIS2, dIR2, rs, tq);
+    const float qc = clamp_ns(q, nsf);
+    if (!fast_ambiguous(qc, EPS)) return fast_T(qc);
+    if (q - nsf >= __builtin_fmaf(q, EPS, EPS)) return ns;
+    return exact_T(dIS2, dIR2, t2, v2, 1.0f / v2, dt, rdt, ns);
+}
+
 /* One workgroup = 8 waves = NCOL image columns x W time segments.  Each lane
  * owns K samples t = seg*64K + 64k + lane of its column: the image values
- * (km, fold count, Kahan compensation), t2[t] and 1/v^2 stay in registers
- * for the whole launch, v^2 in LDS.  Traces stream through LDS in trace
- * order: a chunk of 512 is filtered to those whose CMP can reach the tile
- * (ordered compaction), then staged 8 at a time; per staged trace the
- * aperture and slab gates of every column are evaluated lane-parallel
- * (one lane per trace) and the passing traces are walked in order.
+ * (km, fold count, Kahan compensation) and the fast path's rs = RN(1/(v^2
+ * dt^2)) and tq = t^2/4 stay in registers for the whole launch.  Traces
+ * stream through LDS in trace order: a chunk of 512 is filtered to those
+ * whose CMP can reach the tile (ordered compaction), then staged 8 at a
+ * time; per staged trace the aperture and slab gates of every column are
+ * evaluated lane-parallel (one lane per trace) and the passing traces are
+ * walked in order.
  *
- * FASTIDX = false: the reference's traveltime index, exactly: the fast form
- *   plus the reference arithmetic (exact_T) wherever an integer lies within
- *   the fast form's error bound of q.  q is clamped at ns, so a sample at or
- *   past the end of the trace lands on an integer and takes that rare path,
- *   where the samples that do not count towards fold are counted: the
- *   common path has no compare for fold.
- * FASTIDX = true: T = floor(min(q, ns)) of the fast form alone, for the
- *   CLOSED and 16-bit modes whose contract is an error bound against FP64
- *   rather than the reference's bits: no ambiguity test, no exact path, no
- *   v^2 in LDS.  Fold is summed from a row V[T] = (T < ns) staged once in
- *   LDS (an address add and a float add per sample instead of a compare
- *   and a carry add, both quarter-rate on gfx950).
- * The plain variants fit 128 VGPRs (4 waves per SIMD, two workgroups per CU,
- * one staging while the other computes). */
+ * Traveltime index, three variants:
+ * - plain sums (the FP32 reference path): the reference's index, exactly,
+ *   resolved speculatively.  The fast form's floor(q) is used at once; a
+ *   sample whose q lies within the form's error bound of an integer (and is
+ *   not past the trace for certain) is added with that index and queued in
+ *   LDS with the sum before the add (one per lane and batch; a lane's
+ *   further ambiguous samples take the inline exact path).  At the end of the staged batch one
+ *   wave-wide pass computes the reference index of every lane's queued
+ *   sample together (the exact arithmetic used to run for the whole wave
+ *   per ambiguous lane and slot); where it differs from the fast one (a few
+ *   percent of them) the sample's sum is rebuilt from the saved value
+ *   through the batch's later traces in order, so every sample receives its
+ *   additions in the reference's order and the image is bit-identical.
+ * - Kahan sums, exact index: the reference arithmetic inline for the
+ *   ambiguous samples.
+ * Both exact variants keep v^2 of the column's samples in LDS.
+ * - FASTIDX: T = floor(min(q, ns)) of the fast form alone (CLOSED and 16-bit
+ *   modes, whose contract is an error bound against FP64): no ambiguity
+ *   test; fold is summed from a row V[T] = (T < ns) staged once in LDS (an
+ *   address add and a float add per sample instead of a compare and a carry
+ *   add, both quarter-rate on gfx950).
+ * q is clamped at ns, so in the exact variants a sample at or past the end of
+ * the trace lands on an integer and takes the rare branch, where the samples
+ * that do not count towards fold are counted: the common path has no compare
+ * for fold.  The plain variants fit 128 VGPRs (4 waves per SIMD, two
+ * workgroups per CU, one staging while the other computes). */
 template <int K, int W, int B, bool KAHAN, bool FASTIDX>
 __global__ void __launch_bounds__(8 * KTM_WAVE) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_migrate(const MigArgs a)
@@ -643,15 +675,20 @@ k_migrate(const MigArgs a)
     constexpr int NCOL = 8 / W;             /* image columns per workgroup */
     constexpr int NW = NCOL * W;            /* waves per workgroup (8) */
     constexpr int NT = NW * KTM_WAVE;
+    constexpr bool SPEC = !KAHAN && !FASTIDX;
+    static_assert(B <= 16 && K <= 16, "queue items pack the slot and the trace in 4 bits each");
     extern __shared__ float4 lds4[];
     const int ldl = (int)a.row;             /* LDS row = prepared row (floats) */
     float *sF = reinterpret_cast<float *>(lds4);            /* [B][ldl] */
     float *sG = sF + B * ldl;                               /* [B][8]   */
     int *sSel = reinterpret_cast<int *>(sG + B * 8);        /* [NT]     */
     int *sCnt = sSel + NT;                                  /* [NW]     */
-    /* exact index: v^2 per sample, [NCOL][W*K*64]; fast index: V[T] = (T < ns),
-     * [ldl] */
+    /* exact index: v^2 per sample [NCOL][W*K*64], then (plain sums) the
+     * speculation queue [NW][KTM_SPEC_Q][64] of uint2 (sum before the add,
+     * slot | trace << 4 | index << 8 | mismatch << 31: the fast index when
+     * queued, the reference's once resolved); fast index: V[T], [ldl] */
     float *sV2 = reinterpret_cast<float *>(sCnt + NW);
+    uint2 *sQ = reinterpret_cast<uint2 *>(sV2 + NCOL * (W * K * KTM_WAVE));
 
     const int tid = threadIdx.x, lane = tid & (KTM_WAVE - 1), wv = tid / KTM_WAVE;
     /* XCD-aware tile order: consecutive workgroup ids go to different XCDs.
@@ -687,20 +724,15 @@ k_migrate(const MigArgs a)
     const float reach = a.aperture * 1.0001f + sqrtf(hx * hx + hy * hy) + 1.0f;
     const float reach2 = reach * reach;
 
-    /* v^2 of this column for the exact path lives in LDS (t2 is read from
-     * the table there); the fast path (fast_qs: radicands pre-scaled by
-     * 1/dt^2) keeps rs = RN(1/(v^2 dt^2)) and tq = t^2/4 in registers.  The
-     * plain exact-index variants keep RN(1/v^2) for the exact path in
-     * registers too; the Kahan variants, whose compensation takes K
-     * registers, divide for it on the exact path. */
+    /* exact variants: v^2 of this column's samples in LDS for the exact path */
     float *sV2c = sV2 + cw * (W * K * KTM_WAVE);
     if (FASTIDX) {
         for (int c = tid; c < ldl; c += NT) sV2[c] = c < ns ? 1.0f : 0.0f;
     }
+    uint2 *sQw = sQ + wv * (KTM_SPEC_Q * KTM_WAVE);
     const int64_t pix = (((int64_t)Xc * a.nyp + Yc) * a.nh + h) * ns;
     const int64_t vbase = ((int64_t)a.velx[Xc] * a.nvel_xl + a.vely[Yc]) * a.nsm;
-    constexpr bool RREG = !KAHAN && !FASTIDX;   /* RN(1/v^2) per slot in registers */
-    float r[RREG ? K : 1], rs[K], tq[K], acc[K], cmp[K];
+    float rs[K], tq[K], acc[K], cmp[K];
     int cnt[FASTIDX ? 1 : K];               /* passing traces with T >= ns, per slot */
     float vcnt[FASTIDX ? K : 1];            /* passing traces with T < ns, per slot */
 #pragma unroll
@@ -712,7 +744,6 @@ k_migrate(const MigArgs a)
         if (tl < ns) {
             const float v2 = a.vel2[vbase + a.t1[tl]];
             if (!FASTIDX) sV2c[tl] = v2;
-            if (RREG) r[RREG ? k : 0] = 1.0f / v2;
             rs[k] = fast_rs(v2, a.dt);
             tq[k] = fast_tq(tl);
             acc[k] = col_ok ? a.km[pix + tl] : 0.0f;
@@ -721,7 +752,6 @@ k_migrate(const MigArgs a)
             if (!FASTIDX) sV2c[tl] = 1.0f;
             /* padding lane: q = 0.5, never flagged by the ambiguity test; it
              * reads F[0] (and V[0]) into values that are not stored */
-            if (RREG) r[RREG ? k : 0] = 0.0f;
             rs[k] = 0.0f;
             tq[k] = 0.0625f;
             acc[k] = 0.0f;
@@ -746,6 +776,14 @@ k_migrate(const MigArgs a)
             y[k] = yk - (s - acc[k]);           /* -c, c = (s - acc) - y */
             acc[k] = s;
         }
+    };
+    /* the squared source and receiver distances of staged trace j from this
+     * column, as the gate evaluation forms them */
+    auto staged_dist = [&](int j, float &dIS2, float &dIR2) __attribute__((always_inline)) {
+        const float4 g0 = reinterpret_cast<const float4 *>(sG + j * 8)[0];
+        const float ax = IX - g0.x, ay = IY - g0.y, bx = IX - g0.z, by = IY - g0.w;
+        dIS2 = ax * ax + ay * ay;
+        dIR2 = bx * bx + by * by;
     };
 
     uint32_t npass = 0;
@@ -797,20 +835,19 @@ k_migrate(const MigArgs a)
                 float g_is = 0.0f, g_ir = 0.0f;
                 bool gp = false;
                 if (lane < nb) {
-                    const float4 g0 = reinterpret_cast<const float4 *>(sG + lane * 8)[0];
+                    staged_dist(lane, g_is, g_ir);
                     const float4 g1 = reinterpret_cast<const float4 *>(sG + lane * 8)[1];
-                    const float ax = IX - g0.x, ay = IY - g0.y, bx = IX - g0.z, by = IY - g0.w;
                     const float cx = IX - g1.x, cy = IY - g1.y;
                     const float ex = scx - g1.x, ey = scy - g1.y;
-                    g_is = ax * ax + ay * ay;
-                    g_ir = bx * bx + by * by;
                     gp = ex * ex + ey * ey < sthr && cx * cx + cy * cy < a.ap_x2thr;
                 }
-                uint64_t pm = __ballot(gp);
+                const uint64_t pm_batch = __ballot(gp);
+                uint64_t pm = pm_batch;
                 npass += __popcll(pm);
+                int qn = 0;                         /* queued samples of this lane (plain) */
                 /* Hot loop, Kirchhoff.c:1374-1399: the tap F[T] of slot k for
                  * one trace (and, fast index, V[T] for fold). */
-                auto tap = [&](int k, float dIS2, float dIR2, const float *Fr) __attribute__((always_inline)) {
+                auto tap = [&](int k, int j, float dIS2, float dIR2, const float *Fr) __attribute__((always_inline)) {
                     const float q = fast_qs(dIS2, dIR2, rs[k], tq[k]);
                     const float qc = clamp_ns(q, nsf);
                     int T = fast_T(qc);
@@ -822,15 +859,21 @@ k_migrate(const MigArgs a)
                     }
                     constexpr float EPS = KTM_FAST_EPS_S;
                     if (fast_ambiguous(qc, EPS)) {
-                        if (q - nsf >= __builtin_fmaf(q, EPS, EPS))
+                        if (q - nsf >= __builtin_fmaf(q, EPS, EPS)) {
                             T = ns;   /* Q >= ns as well: past the trace */
-                        else {
+                            cnt[FASTIDX ? 0 : k] += 1;
+                        } else if (SPEC && qn < KTM_SPEC_Q) {
+                            /* speculate: floor(q) now, the reference index at
+                             * the end of the batch */
+                            sQw[qn * KTM_WAVE + lane] = make_uint2(
+                                __float_as_uint(acc[k]), (uint32_t)(k | (j << 4) | (T << 8)));
+                            ++qn;
+                        } else {
                             const int tl = seg * K * KTM_WAVE + k * KTM_WAVE + lane;
                             const float v2 = sV2c[tl];
-                            T = exact_T(dIS2, dIR2, a.t2[tl], v2,
-                                        RREG ? r[RREG ? k : 0] : 1.0f / v2, a.dt, a.rdt, ns);
+                            T = exact_T(dIS2, dIR2, a.t2[tl], v2, 1.0f / v2, a.dt, a.rdt, ns);
+                            cnt[FASTIDX ? 0 : k] += T >= ns ? 1 : 0;
                         }
-                        cnt[FASTIDX ? 0 : k] += T >= ns ? 1 : 0;
                     }
                     return Fr[T];
                 };
@@ -842,9 +885,71 @@ k_migrate(const MigArgs a)
                     const float *Fr = sF + j * ldl;
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
-                        const float f = tap(k, dIS2, dIR2, Fr);
+                        const float f = tap(k, j, dIS2, dIR2, Fr);
                         if (KAHAN) y[KAHAN ? k : 0] = y[KAHAN ? k : 0] + f;
                         else acc[k] = acc[k] + f;
+                    }
+                }
+                if (SPEC && __ballot(qn > 0)) {
+                    /* ---- resolve the queued samples: one pass per queue
+                     * depth, every lane's item of that depth together ---- */
+                    bool redo = false;
+                    for (int r = 0; r < KTM_SPEC_Q; ++r) {
+                        if (!__ballot(qn > r)) break;
+                        if (qn > r) {
+                            uint2 e = sQw[r * KTM_WAVE + lane];
+                            const int k = (int)(e.y & 15u), j = (int)((e.y >> 4) & 15u);
+                            const int tl = seg * K * KTM_WAVE + k * KTM_WAVE + lane;
+                            const int Tf = (int)((e.y >> 8) & 0x7fffu);
+                            float dIS2, dIR2;
+                            staged_dist(j, dIS2, dIR2);
+                            const float v2 = sV2c[tl];
+                            const int Te = exact_T(dIS2, dIR2, a.t2[tl], v2, 1.0f / v2, a.dt,
+                                                   a.rdt, ns);
+                            if (Te >= ns) {
+#pragma unroll
+                                for (int kk = 0; kk < K; ++kk)
+                                    if (kk == k) cnt[FASTIDX ? 0 : kk] += 1;
+                            }
+                            e.y = (e.y & 0xffu) | ((uint32_t)Te << 8) |
+                                  (Te != Tf ? 0x80000000u : 0u);
+                            sQw[r * KTM_WAVE + lane] = e;
+                            redo = redo || Te != Tf;
+                        }
+                    }
+                    /* ---- rebuild the sums whose queued index was wrong: from
+                     * the sum before the earliest wrong one, through the
+                     * batch's later passing traces in order (rare) ---- */
+                    if (__ballot(redo) && redo) {
+                        uint32_t done = 0;              /* slots rebuilt */
+                        for (int r = 0; r < qn; ++r) {
+                            const uint2 e = sQw[r * KTM_WAVE + lane];
+                            const int k = (int)(e.y & 15u), j = (int)((e.y >> 4) & 15u);
+                            if (!(e.y & 0x80000000u) || (done >> k & 1u)) continue;
+                            done |= 1u << k;
+                            const int tl = seg * K * KTM_WAVE + k * KTM_WAVE + lane;
+                            const float v2 = sV2c[tl];
+                            float rsk = 0.0f;
+#pragma unroll
+                            for (int kk = 0; kk < K; ++kk)
+                                if (kk == k) rsk = rs[kk];
+                            const float tqk = fast_tq(tl), t2k = a.t2[tl];
+                            float s = __uint_as_float(e.x) +
+                                      sF[j * ldl + (int)((e.y >> 8) & 0x7fffu)];
+                            uint64_t later = pm_batch & ~((2ull << j) - 1ull);
+                            while (later) {
+                                const int jj = __builtin_ctzll(later);
+                                later &= later - 1ull;
+                                float dIS2, dIR2;
+                                staged_dist(jj, dIS2, dIR2);
+                                const int T = reference_T(dIS2, dIR2, rsk, tqk, t2k, v2, a.dt,
+                                                          a.rdt, ns);
+                                s = s + sF[jj * ldl + T];
+                            }
+#pragma unroll
+                            for (int kk = 0; kk < K; ++kk)
+                                if (kk == k) acc[kk] = s;
+                        }
                     }
                 }
                 if (KAHAN && ++nfold == KTM_KAHAN_FOLD) {
@@ -2047,12 +2152,15 @@ static int migrate_block(const ktm_plan *p, const void *prepared, int dtype, int
     const int tiles_y = (a.ry1 - a.ry0 + a.ty_cols - 1) / a.ty_cols;
     a.ntiles = a.tiles_x * tiles_y;
     const int nthreads = 8 * KTM_WAVE;
-    /* staged rows + geometry, selection list and counts, and v^2 per sample
-     * (exact index) or the fold row V (fast index) */
+    /* staged rows + geometry, selection list and counts, and: the
+     * speculation queue (plain exact index), v^2 per sample (Kahan exact
+     * index) or the fold row V (fast index) */
+    const bool kahan = acc_mode == KTM_ACCUM_KAHAN, fast = (accum & KTM_INDEX_FAST) != 0;
+    const size_t tail = fast ? sizeof(float) * (size_t)p->row
+                             : sizeof(float) * (size_t)(8 * K * KTM_WAVE) +
+                                   (kahan ? 0 : sizeof(uint2) * (size_t)(8 * KTM_SPEC_Q * KTM_WAVE));
     const size_t lds = sizeof(float) * (size_t)(B * p->row + B * 8) +
-                       sizeof(int) * (size_t)(nthreads + 8) +
-                       sizeof(float) * ((accum & KTM_INDEX_FAST) ? (size_t)p->row
-                                                                 : (size_t)(8 * K * KTM_WAVE));
+                       sizeof(int) * (size_t)(nthreads + 8) + tail;
     if (lds > 160 * 1024) return KTM_ERR_UNSUPPORTED;
     /* runs of a row of tiles, but at least 8 runs per XCD */
     a.run = a.ntiles / 64 < a.tiles_x ? a.ntiles / 64 : a.tiles_x;
