@@ -34,10 +34,10 @@ BYTES_FP32 = 12.0   # three FP32 trace taps per contribution
 BYTES_HALF = 6.0    # three 16-bit taps
 METRIC = "G trace×image-pt contributions/s, %HBM roofline; BF16-trace vs FP32 at err"
 # HBM bytes per launch of k_migrate on this workload, from rocprofv3 --pmc
-# FETCH_SIZE (450,847,687 KiB, scaled x2.0) + WRITE_SIZE (1,881,311 KiB, x1.0),
+# FETCH_SIZE (437,011,974 KiB, scaled x2.0) + WRITE_SIZE (1,881,561 KiB, x1.0),
 # the scale factors being what the same counters read for a 1 GiB device copy
 # (profiles/k_migrate_area3d_1e6.md)
-TRAFFIC = 925_262_524_992
+TRAFFIC = 896_927_240_448
 
 
 def log(*a):
@@ -176,12 +176,18 @@ def valu_probe(dev, modes=tuple(VALU_MODES), waves_per_simd=(8,), target_ms=40.0
 # v_min/v_med3, v_rndne, v_cmp, v_cndmask, v_readlane are in no class.
 KMIG_COUNTS = {
     "exact": {"kernel": "k_migrate<12, 2, 8, false, false>", "contributions": 1806379368000,
-              "launch_ms": 997.86, "valu": 514084083419, "fma_f32": 98855645105,
-              "add_f32": 96792681245, "mul_f32": 7096351279, "trans_f32": 57809899776,
-              "cvt": 34890724372, "int32": 56955079786, "fma_f64": 20902690694,
-              "add_f64": 1491403621, "mul_f64": 5980974484, "trans_f64": 2986647242,
-              "int64": 5715074464, "salu": 105936930242, "lds": 34905870923,
-              "clock_ghz": 2.383},
+              "launch_ms": 958.56, "clock_ghz": 2.382, "valu": 488628380575,
+              "fma_f32": 94969484010, "add_f32": 97736566828, "mul_f32": 5946021325,
+              "trans_f32": 58496827853, "cvt": 30618638965, "int32": 61308780164,
+              "fma_f64": 5207598064, "add_f64": 370325576, "mul_f64": 1532187535,
+              "trans_f64": 744491152, "int64": 6386285760, "salu": 132365233677,
+              "lds": 36591172744},
+    "fast_kahan": {"kernel": "k_migrate<12, 2, 8, true, true>", "contributions": 1806379368000,
+                   "launch_ms": 748.03, "clock_ghz": 2.377, "valu": 371632398974,
+                   "fma_f32": 57804779776, "add_f32": 93506701952, "mul_f32": 2618300416,
+                   "trans_f32": 57806059776, "cvt": 28921269888, "int32": 81815267442,
+                   "fma_f64": 23040000, "add_f64": 0, "mul_f64": 15360000, "trans_f64": 3840000,
+                   "int64": 5722754464, "salu": 44813163964, "lds": 62309817190},
 }
 # the probe mode whose measured rate prices each class (SIMD cycles per
 # wave-instruction); INT32 at the v_add_u32 rate and the unclassified rest
@@ -862,6 +868,10 @@ def main():
                            "avg_launch_ms": round(float(np.mean(ms)), 2),
                            "trace_bytes_resident": int(sv.ntr * plan.row *
                                                        (2 if store in ("BF16", "F16") else 4))}
+            if label == "bf16_fast":
+                rf = valu_roofline(KMIG_COUNTS["fast_kahan"], probe, n / len(ms),
+                                   float(np.mean(ms)), ncu, KMIG_COUNTS["fast_kahan"]["clock_ghz"])
+                legs[label]["valu_roofline_frac"] = rf["frac"]
         blk, geom0 = plan.prep(traces[:args.block], coords[:args.block], PREP_CLOSED, TRACE_F32)
         legs["index_mismatch"] = index_mismatch(sv, plan, geom0, dev)
         del blk, geom0
