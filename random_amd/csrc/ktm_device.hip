@@ -642,22 +642,21 @@ __device__ __forceinline__ int reference_T(float dIS2, float dIR2, float rs, flo
  * evaluated lane-parallel (one lane per trace) and the passing traces are
  * walked in order.
  *
- * Traveltime index, three variants:
- * - plain sums (the FP32 reference path): the reference's index, exactly,
- *   resolved speculatively.  The fast form's floor(q) is used at once; a
- *   sample whose q lies within the form's error bound of an integer (and is
- *   not past the trace for certain) is added with that index and queued in
- *   LDS with the sum before the add (one per lane and batch; a lane's
- *   further ambiguous samples take the inline exact path).  At the end of the staged batch one
- *   wave-wide pass computes the reference index of every lane's queued
- *   sample together (the exact arithmetic used to run for the whole wave
- *   per ambiguous lane and slot); where it differs from the fast one (a few
- *   percent of them) the sample's sum is rebuilt from the saved value
- *   through the batch's later traces in order, so every sample receives its
- *   additions in the reference's order and the image is bit-identical.
- * - Kahan sums, exact index: the reference arithmetic inline for the
- *   ambiguous samples.
- * Both exact variants keep v^2 of the column's samples in LDS.
+ * Traveltime index, two variants:
+ * - exact (the FP32 reference path, and the Kahan modes unless FASTIDX): the
+ *   reference's index, resolved speculatively.  The fast form's floor(q) is
+ *   used at once; a sample whose q lies within the form's error bound of an
+ *   integer (and is not past the trace for certain) is added with that index
+ *   and queued in LDS with the running sum before the add (one per lane and
+ *   batch; a lane's further ambiguous samples take the inline exact path).
+ *   At the end of the staged batch one wave-wide pass computes the reference
+ *   index of every lane's queued sample together (the exact arithmetic used
+ *   to run for the whole wave per ambiguous lane and slot); where it differs
+ *   from the fast one (a few percent of them) the sample's running sum is
+ *   rebuilt from the saved value through the batch's later traces in order,
+ *   so every sample receives its additions in the reference's order and the
+ *   result equals the inline exact path's bit for bit.  v^2 of the column's
+ *   samples stays in LDS for the reference arithmetic.
  * - FASTIDX: T = floor(min(q, ns)) of the fast form alone (CLOSED and 16-bit
  *   modes, whose contract is an error bound against FP64): no ambiguity
  *   test; fold is summed from a row V[T] = (T < ns) staged once in LDS (an
@@ -666,8 +665,10 @@ __device__ __forceinline__ int reference_T(float dIS2, float dIR2, float rs, flo
  * q is clamped at ns, so in the exact variants a sample at or past the end of
  * the trace lands on an integer and takes the rare branch, where the samples
  * that do not count towards fold are counted: the common path has no compare
- * for fold.  The plain variants fit 128 VGPRs (4 waves per SIMD, two
- * workgroups per CU, one staging while the other computes). */
+ * for fold.  Every variant fits 128 VGPRs (4 waves per SIMD, two workgroups
+ * per CU, one staging while the other computes; the Kahan exact variant
+ * spills a few 64-bit addresses that are reloaded once per chunk of 512
+ * traces). */
 template <int K, int W, int B, bool KAHAN, bool FASTIDX>
 __global__ void __launch_bounds__(8 * KTM_WAVE) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_migrate(const MigArgs a)
@@ -675,7 +676,7 @@ k_migrate(const MigArgs a)
     constexpr int NCOL = 8 / W;             /* image columns per workgroup */
     constexpr int NW = NCOL * W;            /* waves per workgroup (8) */
     constexpr int NT = NW * KTM_WAVE;
-    constexpr bool SPEC = !KAHAN && !FASTIDX;
+    constexpr bool SPEC = !FASTIDX;
     static_assert(B <= 16 && K <= 16, "queue items pack the slot and the trace in 4 bits each");
     extern __shared__ float4 lds4[];
     const int ldl = (int)a.row;             /* LDS row = prepared row (floats) */
@@ -683,8 +684,8 @@ k_migrate(const MigArgs a)
     float *sG = sF + B * ldl;                               /* [B][8]   */
     int *sSel = reinterpret_cast<int *>(sG + B * 8);        /* [NT]     */
     int *sCnt = sSel + NT;                                  /* [NW]     */
-    /* exact index: v^2 per sample [NCOL][W*K*64], then (plain sums) the
-     * speculation queue [NW][KTM_SPEC_Q][64] of uint2 (sum before the add,
+    /* exact index: v^2 per sample [NCOL][W*K*64], then the speculation
+     * queue [NW][KTM_SPEC_Q][64] of uint2 (running sum before the add,
      * slot | trace << 4 | index << 8 | mismatch << 31: the fast index when
      * queued, the reference's once resolved); fast index: V[T], [ldl] */
     float *sV2 = reinterpret_cast<float *>(sCnt + NW);
@@ -866,7 +867,8 @@ k_migrate(const MigArgs a)
                             /* speculate: floor(q) now, the reference index at
                              * the end of the batch */
                             sQw[qn * KTM_WAVE + lane] = make_uint2(
-                                __float_as_uint(acc[k]), (uint32_t)(k | (j << 4) | (T << 8)));
+                                __float_as_uint(KAHAN ? y[KAHAN ? k : 0] : acc[k]),
+                                (uint32_t)(k | (j << 4) | (T << 8)));
                             ++qn;
                         } else {
                             const int tl = seg * K * KTM_WAVE + k * KTM_WAVE + lane;
@@ -947,8 +949,11 @@ k_migrate(const MigArgs a)
                                 s = s + sF[jj * ldl + T];
                             }
 #pragma unroll
-                            for (int kk = 0; kk < K; ++kk)
-                                if (kk == k) acc[kk] = s;
+                            for (int kk = 0; kk < K; ++kk) {
+                                if (kk != k) continue;
+                                if (KAHAN) y[KAHAN ? kk : 0] = s;
+                                else acc[kk] = s;
+                            }
                         }
                     }
                 }
@@ -2152,13 +2157,12 @@ static int migrate_block(const ktm_plan *p, const void *prepared, int dtype, int
     const int tiles_y = (a.ry1 - a.ry0 + a.ty_cols - 1) / a.ty_cols;
     a.ntiles = a.tiles_x * tiles_y;
     const int nthreads = 8 * KTM_WAVE;
-    /* staged rows + geometry, selection list and counts, and: the
-     * speculation queue (plain exact index), v^2 per sample (Kahan exact
-     * index) or the fold row V (fast index) */
-    const bool kahan = acc_mode == KTM_ACCUM_KAHAN, fast = (accum & KTM_INDEX_FAST) != 0;
+    /* staged rows + geometry, selection list and counts, and: v^2 per sample
+     * and the speculation queue (exact index) or the fold row V (fast index) */
+    const bool fast = (accum & KTM_INDEX_FAST) != 0;
     const size_t tail = fast ? sizeof(float) * (size_t)p->row
                              : sizeof(float) * (size_t)(8 * K * KTM_WAVE) +
-                                   (kahan ? 0 : sizeof(uint2) * (size_t)(8 * KTM_SPEC_Q * KTM_WAVE));
+                                   sizeof(uint2) * (size_t)(8 * KTM_SPEC_Q * KTM_WAVE);
     const size_t lds = sizeof(float) * (size_t)(B * p->row + B * 8) +
                        sizeof(int) * (size_t)(nthreads + 8) + tail;
     if (lds > 160 * 1024) return KTM_ERR_UNSUPPORTED;
