@@ -11,8 +11,9 @@
  * Pointers documented as "device" are HIP device memory; `stream` is a
  * hipStream_t passed as void* (NULL = the null stream).  A plan is not
  * thread-safe; distinct plans may be used from distinct threads.  A plan's
- * workspaces (the BF16/FP16 decode buffer of ktm_migrate_block, the
- * demigration sort keys and max-|km| word of ktm_demigrate) are reused by
+ * workspaces (the BF16/FP16 decode buffer and the compensated partial
+ * images of ktm_migrate_block, the demigration sort keys and max-|km| word
+ * of ktm_demigrate) are reused by
  * every call without stream ordering: all calls on one plan must be issued
  * on a single stream (or the caller orders them), or a later call can
  * overwrite a workspace an earlier call's kernels still read.
@@ -195,7 +196,12 @@ int ktm_image_reset(const ktm_plan *plan, float *km, float *fold, float *comp,
  * km/fold/comp are [nxp][nyp][nh][ns]; comp is needed for KAHAN only.
  * Accumulation continues from the current contents, in trace order, so
  * successive blocks equal one pass over their concatenation (bit for bit
- * with PLAIN; with KAHAN within the compensated bound).  If counter is
+ * with PLAIN; with KAHAN within the compensated bound).  With KAHAN a block
+ * whose image region is small (fewer than 10240 workgroups of tiles x
+ * offset classes) is split along its traces into up to 32 parts summed
+ * separately and merged in FP64 (deterministic: the same call gives the
+ * same bits); the partial images live in the plan, allocated by
+ * ktm_plan_create for an image that small (at most 1 GiB).  If counter is
  * non-NULL (device uint64) the number of contributions is added to it: ns per
  * (trace, column) pair through the aperture gates of Kirchhoff.c:1338,1354,
  * traces outside the offset classes included (the reference evaluates their
@@ -215,7 +221,9 @@ int ktm_migrate_block(const ktm_plan *plan, const void *prepared, int trace_dtyp
  * use, is instead a fixed device array of KTM_STATIC_STAGE_FLOATS floats
  * (64 MiB) that each GPU holds until the process ends, so a BF16/FP16 call
  * takes at most KTM_STATIC_STAGE_FLOATS / ktm_plan_trace_row(plan) traces
- * (KTM_ERR_UNSUPPORTED beyond; FP32 storage needs no workspace).  Every call
+ * (KTM_ERR_UNSUPPORTED beyond; FP32 storage needs no workspace).  The
+ * compensated partial images were allocated by ktm_plan_create and serve
+ * both entry points alike.  Every call
  * on one GPU shares that array: calls that overlap on a GPU (other streams,
  * other plans, other threads) race and must be serialised by the caller. */
 #define KTM_STATIC_STAGE_FLOATS (16u << 20)

@@ -73,9 +73,14 @@ struct ktm_plan {
     size_t dm_cap, dm_tmp_bytes;
     uint32_t *d_dm_amax;    /* bits of max |km| of the last demigration */
     float *d_scratch;       /* this device's copy of `scratch` (ktm_migrate_block_static) */
+    /* compensated migration of small images: partial images of parts 1..
+     * parts_max-1 of a trace block, [part][km, fold, comp][image or region] */
+    float *d_parts;
+    int parts_max;
 };
 
 static int scratch_address(int dev, float **out);
+static int parts_workspace(ktm_plan *p);
 
 #define HIP_TRY(expr)                                  \
     do {                                               \
@@ -520,6 +525,12 @@ struct MigArgs {
     int ns, nh, nxp, nyp, nx, ny, jobs_y, nvel_xl, nsm;
     int tiles_x, ntiles, tx_cols, ty_cols, run;
     int rx0, ry0, rx1, ry1;          /* migrated region [rx0,rx1) x [ry0,ry1) */
+    /* Kahan variants: the block's traces in parts of part_len (a multiple of
+     * the 512-trace chunk), part = blockIdx.z; part 0 adds into km/fold/comp,
+     * part p > 0 writes its sums into parts + (p-1) * 3 * part_n, region
+     * layout [X-rx0][Y-ry0][h][t] */
+    int64_t part_len, part_n;
+    float *parts;
 };
 
 /* Reference semantics of one traveltime index (Kirchhoff.c:1381-1384):
@@ -702,6 +713,7 @@ k_migrate(const MigArgs a)
     const int tile = (((wi / a.run) << 3) + (int)(blockIdx.x & 7)) * a.run + wi % a.run;
     if (tile >= a.ntiles) return;
     const int h = blockIdx.y;
+    const int part = KAHAN ? (int)blockIdx.z : 0;
     const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
     const int cw = wv / W, seg = wv % W;    /* column within tile, time segment */
     const int X = a.rx0 + tx * a.tx_cols + cw % a.tx_cols;
@@ -747,8 +759,8 @@ k_migrate(const MigArgs a)
             if (!FASTIDX) sV2c[tl] = v2;
             rs[k] = fast_rs(v2, a.dt);
             tq[k] = fast_tq(tl);
-            acc[k] = col_ok ? a.km[pix + tl] : 0.0f;
-            if (KAHAN && col_ok) cmp[k] = a.comp[pix + tl];
+            acc[k] = col_ok && part == 0 ? a.km[pix + tl] : 0.0f;
+            if (KAHAN && col_ok && part == 0) cmp[k] = a.comp[pix + tl];
         } else {
             if (!FASTIDX) sV2c[tl] = 1.0f;
             /* padding lane: q = 0.5, never flagged by the ambiguity test; it
@@ -789,11 +801,13 @@ k_migrate(const MigArgs a)
 
     uint32_t npass = 0;
     const int n4 = ldl / 4;
-    for (int64_t c0 = 0; c0 < a.ntr; c0 += NT) {
+    const int64_t cbeg = KAHAN ? part * a.part_len : 0;
+    const int64_t cend = KAHAN ? min(a.ntr, cbeg + a.part_len) : a.ntr;
+    for (int64_t c0 = cbeg; c0 < cend; c0 += NT) {
         /* ---- select the traces of this chunk that can reach the tile, in order ---- */
         const int64_t i = c0 + tid;
         bool pred = false;
-        if (i < a.ntr) {
+        if (i < cend) {
             const ktm_trace_geom g = a.geom[i];
             const float dx = g.cmpx - ccx, dy = g.cmpy - ccy;
             pred = g.off == h && dx * dx + dy * dy <= reach2;
@@ -967,7 +981,24 @@ k_migrate(const MigArgs a)
     }
     if (KAHAN && nfold) fold_y();
 
-    if (col_ok) {
+    if (KAHAN && part > 0 && col_ok) {
+        /* a later part of the block: its own sums, merged by k_merge_parts */
+        float *pk = a.parts + (int64_t)(part - 1) * 3 * a.part_n;
+        const int64_t rpix =
+            (((int64_t)(X - a.rx0) * (a.ry1 - a.ry0) + (Y - a.ry0)) * a.nh + h) * ns;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int t = seg * K * KTM_WAVE + k * KTM_WAVE + lane;
+            if (t < ns) {
+                pk[rpix + t] = acc[k];
+                pk[a.part_n + rpix + t] = FASTIDX ? vcnt[FASTIDX ? k : 0]
+                                                  : (float)((int)npass - cnt[FASTIDX ? 0 : k]);
+                pk[2 * a.part_n + rpix + t] = -y[KAHAN ? k : 0];
+            }
+        }
+        if (lane == 0 && seg == 0 && a.counter)
+            atomicAdd(a.counter, (unsigned long long)npass * (unsigned long long)ns);
+    } else if (col_ok) {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int t = seg * K * KTM_WAVE + k * KTM_WAVE + lane;
@@ -983,6 +1014,34 @@ k_migrate(const MigArgs a)
         }
         if (lane == 0 && seg == 0 && a.counter)
             atomicAdd(a.counter, (unsigned long long)npass * (unsigned long long)ns);
+    }
+}
+
+/* The compensated sums of parts 1..nparts-1 of a trace block added to the
+ * image (part 0 added into it directly): per sample, in FP64, the image's
+ * km - comp and each part's km - comp in part order, rounded once to the new
+ * km, with comp = km - the FP64 total (so km - comp keeps the total to
+ * FP64); fold adds the parts' counts, which are exact.  One thread per
+ * sample of the region. */
+__global__ void k_merge_parts(const MigArgs a, int nparts)
+{
+    const int64_t ht = (int64_t)a.nh * a.ns;
+    const int rny = a.ry1 - a.ry0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.part_n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = i / ht, r = i - c * ht;
+        const int64_t pix = ((int64_t)(a.rx0 + c / rny) * a.nyp + (a.ry0 + c % rny)) * ht + r;
+        double tot = (double)a.km[pix] - (double)a.comp[pix];
+        float f = a.fold[pix];
+        for (int p = 1; p < nparts; ++p) {
+            const float *pk = a.parts + (int64_t)(p - 1) * 3 * a.part_n;
+            tot += (double)pk[i] - (double)pk[2 * a.part_n + i];
+            f += pk[a.part_n + i];
+        }
+        const float k = (float)tot;
+        a.km[pix] = k;
+        a.fold[pix] = f;
+        a.comp[pix] = (float)((double)k - tot);
     }
 }
 
@@ -1537,7 +1596,7 @@ extern "C" void ktm_plan_destroy(ktm_plan *p)
     void *bufs[] = {p->d_t2, p->d_t1, p->d_colx, p->d_coly, p->d_velx, p->d_vely,
                     p->d_slab_cx, p->d_slab_cy, p->d_slab_x2thr, p->d_vel2, p->d_w, p->d_hconv2,
                     p->d_stage, p->d_pix_v2, p->d_pix_rs, p->d_dm_keys, p->d_dm_idx,
-                    p->d_dm_tmp, p->d_dm_amax};
+                    p->d_dm_tmp, p->d_dm_amax, p->d_parts};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     ktm_tables_free(&p->tb);
@@ -1595,6 +1654,7 @@ extern "C" int ktm_plan_create(ktm_plan **out, const ktm_config *cfg, int ns, fl
         if (st == KTM_OK) st = upload(&p->d_hconv2, h2, 2 * (size_t)ns);
         free(h2);
     }
+    if (st == KTM_OK) st = parts_workspace(p);
     if (st != KTM_OK) { ktm_plan_destroy(p); return st; }
     *out = p;
     return KTM_OK;
@@ -2010,6 +2070,61 @@ static mig_fn pick_migrate(int ns, int accum, int64_t wgs8, int *W, int *K, int 
     return pick_accum<12, 8, 5>(accum, B);
 }
 
+/* Tiles of the region [rx0,rx1) x [ry0,ry1) at W waves per column: 8/W
+ * columns, a row on a 2-D line, as square as possible in 3-D. */
+static void tile_layout(const ktm_setup *s, int W, MigArgs *a)
+{
+    const int ncol = 8 / W;
+    a->tx_cols = s->nyp == 1 ? ncol : (ncol == 8 ? 4 : ncol == 4 ? 2 : ncol);
+    a->ty_cols = ncol / a->tx_cols;
+    a->tiles_x = (a->rx1 - a->rx0 + a->tx_cols - 1) / a->tx_cols;
+    const int tiles_y = (a->ry1 - a->ry0 + a->ty_cols - 1) / a->ty_cols;
+    a->ntiles = a->tiles_x * tiles_y;
+}
+
+/* Compensated sums do not need the reference's order of additions, so a
+ * block whose image region gives fewer than 10240 workgroups (20 per
+ * workgroup slot, 256 CUs x 2) is split along its traces into parts, each
+ * part a workgroup per tile with its own sums, merged afterwards.  On the
+ * 1600-column, 1501-sample line (400 tiles) L^T took 33.7 ms unsplit, 27.6
+ * with 7 parts, 25.2 with 12 and 24.2 with 24. */
+#define KTM_PART_TARGET_WGS 10240
+#define KTM_MAX_PARTS 32
+#define KTM_PARTS_MAX_BYTES (1ull << 30)
+
+static int parts_for(int64_t wgs)
+{
+    if (wgs >= KTM_PART_TARGET_WGS) return 1;
+    const int64_t n = (KTM_PART_TARGET_WGS + wgs - 1) / wgs;
+    return n > KTM_MAX_PARTS ? KTM_MAX_PARTS : (int)n;
+}
+
+/* the partial images of the compensated split, sized for the whole image
+ * (a job region needs at most that), allocated once with the plan so that
+ * neither migration entry point allocates them; none for a large image */
+static int parts_workspace(ktm_plan *p)
+{
+    const ktm_setup *s = &p->tb.s;
+    MigArgs a;
+    memset(&a, 0, sizeof a);
+    a.rx1 = s->nxp; a.ry1 = s->nyp;
+    int W = 1, K = 8, B = 8;
+    (void)pick_migrate(s->ns, KTM_ACCUM_KAHAN, ((int64_t)s->nxp * s->nyp + 7) / 8 * s->nh, &W, &K,
+                       &B);
+    tile_layout(s, W, &a);
+    int n = parts_for((int64_t)a.ntiles * s->nh);
+    const uint64_t per = 3ull * sizeof(float) * (uint64_t)s->nxp * s->nyp * s->nh * s->ns;
+    if (n > 1 && (uint64_t)(n - 1) * per > KTM_PARTS_MAX_BYTES)
+        n = 1 + (int)(KTM_PARTS_MAX_BYTES / per);
+    p->parts_max = n > 1 ? n : 1;
+    if (p->parts_max > 1 &&
+        hipMalloc((void **)&p->d_parts, (size_t)(p->parts_max - 1) * per) != hipSuccess) {
+        p->d_parts = nullptr;
+        return KTM_ERR_NOMEM;
+    }
+    return KTM_OK;
+}
+
 /* FP32 filtered traces of a BF16/FP16 block, into stage (ntr * row floats)
  * or, when stage is NULL, into a workspace the plan owns and grows on demand
  * (the only allocation on the migration path) */
@@ -2149,14 +2264,21 @@ static int migrate_block(const ktm_plan *p, const void *prepared, int dtype, int
     int W = 1, K = 8, B = 8;
     const int64_t ncols_region = (int64_t)(a.rx1 - a.rx0) * (a.ry1 - a.ry0);
     mig_fn fn = pick_migrate(s->ns, accum, (ncols_region + 7) / 8 * s->nh, &W, &K, &B);
-    /* tile of 8/W columns: a row on a 2-D line, as square as possible in 3-D */
-    const int ncol = 8 / W;
-    a.tx_cols = s->nyp == 1 ? ncol : (ncol == 8 ? 4 : ncol == 4 ? 2 : ncol);
-    a.ty_cols = ncol / a.tx_cols;
-    a.tiles_x = (a.rx1 - a.rx0 + a.tx_cols - 1) / a.tx_cols;
-    const int tiles_y = (a.ry1 - a.ry0 + a.ty_cols - 1) / a.ty_cols;
-    a.ntiles = a.tiles_x * tiles_y;
+    tile_layout(s, W, &a);
     const int nthreads = 8 * KTM_WAVE;
+    /* compensated sums of a small region: the traces in parts */
+    int nparts = 1;
+    a.part_len = ntr;
+    a.part_n = ncols_region * s->nh * s->ns;
+    a.parts = p->d_parts;
+    if (acc_mode == KTM_ACCUM_KAHAN && p->parts_max > 1) {
+        nparts = parts_for((int64_t)a.ntiles * s->nh);
+        if (nparts > p->parts_max) nparts = p->parts_max;
+        if (nparts > 1) {
+            a.part_len = ((ntr + nparts - 1) / nparts + nthreads - 1) / nthreads * nthreads;
+            nparts = (int)((ntr + a.part_len - 1) / a.part_len);
+        }
+    }
     /* staged rows + geometry, selection list and counts, and: v^2 per sample
      * and the speculation queue (exact index) or the fold row V (fast index) */
     const bool fast = (accum & KTM_INDEX_FAST) != 0;
@@ -2172,9 +2294,14 @@ static int migrate_block(const ktm_plan *p, const void *prepared, int dtype, int
     const int64_t span = 8 * (int64_t)a.run;
     const int64_t nblk = ((int64_t)a.ntiles + span - 1) / span * span;
     if (nblk > 0x7fffffffLL) return KTM_ERR_UNSUPPORTED;
-    const dim3 grid((unsigned)nblk, (unsigned)s->nh);
+    const dim3 grid((unsigned)nblk, (unsigned)s->nh, (unsigned)nparts);
     HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(fn, grid, dim3(nthreads), lds, hs, a);
+    if (nparts > 1) {
+        const int64_t nb = (a.part_n + 255) / 256;
+        hipLaunchKernelGGL(k_merge_parts, dim3((unsigned)(nb < 8192 ? nb : 8192)), dim3(256), 0, hs,
+                           a, nparts);
+    }
     if (counter)
         hipLaunchKernelGGL(k_count_outside, dim3((unsigned)((ntr + 255) / 256)), dim3(256), 0, hs, a);
     HIP_TRY(hipGetLastError());

@@ -221,7 +221,10 @@ def test_compensated_line_vs_fp64(cuda, nh, jobs_x, block):
     400 columns; four offset classes and three job slabs in the second
     case; blocks of 997 traces in the third, so the compensation image is
     carried over nine launches and the staged batches end at every phase
-    of the kernel's four-batch compensated step): the contribution count
+    of the kernel's four-batch compensated step).  The image is small
+    enough for the kernel to split each block along its traces into parts
+    of 512 (17 parts of the 8400-trace block, two of a 997-trace one)
+    merged in FP64 afterwards: the contribution count
     and the fold equal the oracle's exactly and the image is within 1e-6
     of the FP64 oracle."""
     sv = surveys.line2d(nshots=70, ns=501, ncols=400, nscat=8, nh=nh, dh=1500.0 / nh,
