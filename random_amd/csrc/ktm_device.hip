@@ -17,9 +17,10 @@
  *                   tile of image columns (one wave per column, all ns samples
  *                   in registers), streams the traces of a block through LDS in
  *                   trace order and accumulates F(floor(dsr/dt)) per sample
- *   k_postmx<PostKtm>  derivative/fold + low-pass, Kirchhoff.c:1457-1479, as
- *                   an FP64 matrix product on the matrix cores (ktm_postmx.h);
- *                   its transpose k_postmx<PostImg> + k_poststack_adjoint_tail
+ *   k_postfft<PostKtm>  derivative/fold + low-pass, Kirchhoff.c:1457-1479, as
+ *                   an FP64 FFT convolution (ktm_postfft.h; beyond 4096
+ *                   samples the FP64 matrix product of ktm_postmx.h); its
+ *                   transpose k_postfft<PostImg> + k_poststack_adjoint_tail
  *   k_image_*       image<job>.su trace records (headers + big-endian
  *                   samples), Kirchhoff.c:1437-1598
  *   k_demigrate     exact transpose of k_migrate (CLOSED prep)
@@ -42,6 +43,7 @@
 #include "ktm_internal.h"
 #include "ktm_devmath.h"
 #include "ktm_postmx.h"
+#include "ktm_postfft.h"
 
 #define KTM_WAVE 64
 
@@ -62,6 +64,8 @@ struct ktm_plan {
     float *d_vel2;          /* [nvel_il][nvel_xl][nsm], squared like Kirchhoff.c:617 */
     float *d_w;             /* [ns+2] CLOSED weights */
     double *d_hconv2;       /* [2 ns] post-stack kernel, h2[m] = h[m mod ns] */
+    double2 *d_pf_g, *d_pf_tw; /* [N] FFT form of the post-stack kernel and twiddles */
+    int pf_logn;            /* log2 N of the FFT form, 0: the direct product */
     float *d_stage;         /* FP32 filtered traces of a BF16/FP16 block */
     size_t stage_cap;       /* elements of d_stage */
     float *d_pix_v2, *d_pix_rs; /* per-pixel v^2 and RN(1/(v^2 dt^2)), built for demigration */
@@ -1551,7 +1555,8 @@ k_demigrate(const DemigArgs a)
  * 1457-1479 (D: first difference with d[0] = 0, C: -dt^2 t / fold[t],
  * H: the even circular low-pass, P0: out[0] = 0):  km_bar = D^T C H P0 img.
  * H is symmetric (h even), so w = H P0 img is the post-stack convolution of
- * img itself (k_postmx<PostImg>, written into out); this kernel finishes a
+ * img itself (k_postfft<PostImg> or k_postmx<PostImg>, written into out);
+ * this kernel finishes a
  * trace per workgroup in place: uu[t] = C[t] w[t] (FP64, uu[0] = uu[ns] = 0),
  * out[t] = uu[t] - uu[t + 1]. */
 struct PostImg {
@@ -1596,7 +1601,7 @@ extern "C" void ktm_plan_destroy(ktm_plan *p)
     void *bufs[] = {p->d_t2, p->d_t1, p->d_colx, p->d_coly, p->d_velx, p->d_vely,
                     p->d_slab_cx, p->d_slab_cy, p->d_slab_x2thr, p->d_vel2, p->d_w, p->d_hconv2,
                     p->d_stage, p->d_pix_v2, p->d_pix_rs, p->d_dm_keys, p->d_dm_idx,
-                    p->d_dm_tmp, p->d_dm_amax, p->d_parts};
+                    p->d_dm_tmp, p->d_dm_amax, p->d_parts, p->d_pf_g, p->d_pf_tw};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     ktm_tables_free(&p->tb);
@@ -1653,6 +1658,17 @@ extern "C" int ktm_plan_create(ktm_plan **out, const ktm_config *cfg, int ns, fl
         for (int m = 0; st == KTM_OK && m < 2 * ns; ++m) h2[m] = tb->hconv[m < ns ? m : m - ns];
         if (st == KTM_OK) st = upload(&p->d_hconv2, h2, 2 * (size_t)ns);
         free(h2);
+    }
+    if (st == KTM_OK && (p->pf_logn = postfft_logn(ns)) != 0) {
+        const size_t n = (size_t)1 << p->pf_logn;
+        double *g = (double *)malloc(sizeof(double) * 4 * n);
+        if (!g) st = KTM_ERR_NOMEM;
+        if (st == KTM_OK) {
+            postfft_host_tables(tb->hconv, ns, p->pf_logn, g, g + 2 * n);
+            st = upload(&p->d_pf_g, reinterpret_cast<const double2 *>(g), n);
+            if (st == KTM_OK) st = upload(&p->d_pf_tw, reinterpret_cast<const double2 *>(g + 2 * n), n);
+        }
+        free(g);
     }
     if (st == KTM_OK) st = parts_workspace(p);
     if (st != KTM_OK) { ktm_plan_destroy(p); return st; }
@@ -2313,8 +2329,12 @@ extern "C" int ktm_poststack(const ktm_plan *p, const float *km, const float *fo
 {
     if (!p || !km || !fold || !out) return KTM_ERR_ARG;
     const ktm_setup *s = &p->tb.s;
-    const hipError_t e = postmx_launch<PostKtm>(km, fold, (int64_t)s->nxp * s->nyp * s->nh, s->ns,
-                                                s->dt, p->d_hconv2, out, (hipStream_t)stream);
+    const int64_t ntraces = (int64_t)s->nxp * s->nyp * s->nh;
+    const hipError_t e =
+        p->pf_logn ? postfft_launch<PostKtm>(km, fold, ntraces, s->ns, s->dt, p->d_pf_g, p->d_pf_tw,
+                                             out, (hipStream_t)stream)
+                   : postmx_launch<PostKtm>(km, fold, ntraces, s->ns, s->dt, p->d_hconv2, out,
+                                            (hipStream_t)stream);
     return e == hipSuccess ? KTM_OK : e == hipErrorInvalidValue ? KTM_ERR_UNSUPPORTED : KTM_ERR_DEVICE;
 }
 
@@ -2479,8 +2499,11 @@ extern "C" int ktm_poststack_adjoint(const ktm_plan *p, const float *img, const 
     const int64_t ntraces = (int64_t)s->nxp * s->nyp * s->nh;
     if (ntraces > 0x7fffffffLL) return KTM_ERR_UNSUPPORTED;
     hipStream_t hs = (hipStream_t)stream;
-    const hipError_t e = postmx_launch<PostImg>(img, fold, ntraces, s->ns, s->dt, p->d_hconv2,
-                                                km_bar, hs);
+    const hipError_t e =
+        p->pf_logn ? postfft_launch<PostImg>(img, fold, ntraces, s->ns, s->dt, p->d_pf_g,
+                                             p->d_pf_tw, km_bar, hs)
+                   : postmx_launch<PostImg>(img, fold, ntraces, s->ns, s->dt, p->d_hconv2, km_bar,
+                                            hs);
     if (e != hipSuccess) return e == hipErrorInvalidValue ? KTM_ERR_UNSUPPORTED : KTM_ERR_DEVICE;
     hipLaunchKernelGGL(k_poststack_adjoint_tail, dim3((unsigned)ntraces), dim3(PS_NT),
                        sizeof(double) * (size_t)(s->ns + 1), hs, fold, s->ns, s->dt, km_bar);

@@ -21,8 +21,9 @@
  *   latest trace stay in registers until another trace needs their slot.
  *   Traces stream in chunks of 256: the aperture gate of each is evaluated
  *   lane-parallel, the passing ones are compacted in order through LDS.
- * k_postmx<PostEom> (ktm_postmx.h): the post filter as a real even-kernel
- *   circular convolution in FP64 on the matrix cores (the (1+i) factor of
+ * k_postfft<PostEom> (ktm_postfft.h; k_postmx<PostEom> of ktm_postmx.h on
+ *   the matrix cores beyond 4096 samples): the post filter as a real
+ *   even-kernel circular convolution in FP64 (the (1+i) factor of
  *   the sqrt-rho spectrum leaves the real part of a Hermitian product
  *   unchanged), g = IDFT(lowpass * rootrho) / ns.
  * k_eom_records: the bytes of image<job>.su (EOM.c:1346-1537), one thread
@@ -41,6 +42,7 @@
 #include "ktm_internal.h"
 #include "ktm_devmath.h"
 #include "ktm_postmx.h"
+#include "ktm_postfft.h"
 
 #define EOM_NT 256
 #define EOM_RUN 16     /* consecutive tiles per XCD run */
@@ -142,6 +144,8 @@ struct ktm_eom_plan {
     float *d_v2, *d_t2d;      /* [ns] v^2 (EOM.c:877), sqrt(t dt) t dt (EOM.c:1196) */
     double *d_sv;             /* [ns] sqrt(v^2) in double (EOM.c:1290) */
     double *d_g2;             /* [2 ns] post-filter kernel, g2[m] = g[m mod ns] */
+    double2 *d_pf_g, *d_pf_tw; /* [N] FFT form of the post-filter kernel and twiddles */
+    int pf_logn;              /* log2 N of the FFT form, 0: the direct product */
     float4 *d_geo;            /* [cap] per-trace h2, cmpX, cmpY, talpha */
     int64_t geo_cap;
     float v2_0;               /* v^2 at t = 0 (talpha, EOM.c:1272) */
@@ -151,7 +155,8 @@ extern "C" int ktm_eom_plan_destroy(ktm_eom_plan *p)
 {
     if (!p) return KTM_OK;
     (void)hipSetDevice(p->device);
-    void *bufs[] = {p->d_cspx, p->d_cspy, p->d_v2, p->d_t2d, p->d_sv, p->d_g2, p->d_geo};
+    void *bufs[] = {p->d_cspx, p->d_cspy, p->d_v2, p->d_t2d, p->d_sv, p->d_g2, p->d_geo,
+                    p->d_pf_g, p->d_pf_tw};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free(p);
@@ -224,6 +229,22 @@ extern "C" int ktm_eom_plan_create(ktm_eom_plan **out, const ktm_eom_config *c, 
          hipMemcpy(p->d_sv, sv, sizeof(double) * ns, hipMemcpyHostToDevice) != hipSuccess ||
          hipMemcpy(p->d_g2, g, sizeof(double) * 2 * ns, hipMemcpyHostToDevice) != hipSuccess))
         st = KTM_ERR_DEVICE;
+    if (st == KTM_OK && (p->pf_logn = postfft_logn(ns)) != 0) {
+        const size_t n = (size_t)1 << p->pf_logn;
+        double *t = (double *)malloc(sizeof(double) * 4 * n);
+        if (!t) st = KTM_ERR_NOMEM;
+        if (st == KTM_OK) {
+            postfft_host_tables(g, ns, p->pf_logn, t, t + 2 * n);
+            if (hipMalloc(&p->d_pf_g, sizeof(double2) * n) != hipSuccess ||
+                hipMalloc(&p->d_pf_tw, sizeof(double2) * n) != hipSuccess)
+                st = KTM_ERR_NOMEM;
+            else if (hipMemcpy(p->d_pf_g, t, sizeof(double2) * n, hipMemcpyHostToDevice) != hipSuccess ||
+                     hipMemcpy(p->d_pf_tw, t + 2 * n, sizeof(double2) * n, hipMemcpyHostToDevice) !=
+                         hipSuccess)
+                st = KTM_ERR_DEVICE;
+        }
+        free(t);
+    }
     free(lp); free(rr); free(cx); free(cy); free(v2); free(t2d); free(sv); free(g);
     if (st != KTM_OK) {
         ktm_eom_plan_destroy(p);
@@ -437,7 +458,7 @@ __global__ void k_eom_reset(float *csp, float *fold, int64_t n)
 }
 
 /* x of the post filter, EOM.c:1391: the float (csp / fold) dt t, stored into
- * a double; out = g (*) x in FP64 (k_postmx), out[0] = 0 */
+ * a double; out = g (*) x in FP64 (k_postfft or k_postmx), out[0] = 0 */
 struct PostEom {
     static __device__ __forceinline__ double x(const float *c, const float *f, int u, float dt)
     {
@@ -532,8 +553,11 @@ static int eom_post(const ktm_eom_plan *p, const float *csp, const float *fold, 
     const int ns = p->s.ns;
     if (ns > 8192) return KTM_ERR_UNSUPPORTED;
     EOM_TRY(hipSetDevice(p->device));
-    const hipError_t e = postmx_launch<PostEom>(csp, fold, ntraces, ns, p->s.dt, p->d_g2, out,
-                                                (hipStream_t)stream);
+    const hipError_t e =
+        p->pf_logn ? postfft_launch<PostEom>(csp, fold, ntraces, ns, p->s.dt, p->d_pf_g, p->d_pf_tw,
+                                             out, (hipStream_t)stream)
+                   : postmx_launch<PostEom>(csp, fold, ntraces, ns, p->s.dt, p->d_g2, out,
+                                            (hipStream_t)stream);
     return e == hipSuccess ? KTM_OK : e == hipErrorInvalidValue ? KTM_ERR_UNSUPPORTED : KTM_ERR_DEVICE;
 }
 
