@@ -267,6 +267,13 @@ int ktm_poststack(const ktm_plan *plan, const float *km, const float *fold,
  * allocates). */
 int ktm_demigrate(const ktm_plan *plan, const float *km, const ktm_trace_geom *geom,
                   int64_t ntr, float *traces, void *stream);
+/* ktm_demigrate with the traveltime index of KTM_INDEX_FAST (index =
+ * KTM_INDEX_FAST; 0 is ktm_demigrate itself): the exact transpose of
+ * ktm_migrate_block with CLOSED prep and accumulation | KTM_INDEX_FAST, so
+ * that least-squares migration can run on the relaxed index's cheaper pair.
+ * No counterpart in the reference. */
+int ktm_demigrate_index(const ktm_plan *plan, const float *km, const ktm_trace_geom *geom,
+                        int64_t ntr, float *traces, int index, void *stream);
 int ktm_poststack_adjoint(const ktm_plan *plan, const float *image, const float *fold,
                           float *km, void *stream);
 

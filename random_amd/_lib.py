@@ -145,6 +145,7 @@ _PROTOS = {
     "ktm_selftest_sqrt": (_i32, [_vp, _i64, _vp, _vp]),
     "ktm_gate_threshold": (C.c_float, [C.c_double]),
     "ktm_demigrate": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp]),
+    "ktm_demigrate_index": (_i32, [_vp, _vp, _vp, _i64, _vp, _i32, _vp]),
     "ktm_poststack_adjoint": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "ktm_ingest": (_i32, [_vp, _vp, _i64, _i32, _i32, _i32, _vp, _vp, _vp, _vp]),
     "ktm_segy_decode": (_i32, [_vp, _i64, _i32, _i32, _vp, _vp, _vp]),

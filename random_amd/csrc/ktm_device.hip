@@ -1409,8 +1409,11 @@ __global__ void k_demig_keys(const ktm_trace_geom *__restrict__ geom, int64_t nt
  * Lane l owns samples l*K + ((k + l) mod K) of its wave's 64K-sample
  * segment, not k*64 + l: far from the CMP, T(t) grows slowly at early
  * times, and 64 consecutive samples land on a few T, whose atomics then
- * serialise; samples ~13 apart land on distinct T in distinct banks. */
-template <int K, int W>
+ * serialise; samples ~13 apart land on distinct T in distinct banks.
+ * FASTIDX: the index floor(min(q, ns + 1/2)) of the fast form alone, the
+ * transpose of k_migrate's FASTIDX variant (floor(min(q, ns)) there: both
+ * are ns for q >= ns, so they index the same samples). */
+template <int K, int W, bool FASTIDX>
 __global__ void __launch_bounds__(8 * KTM_WAVE) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_demigrate(const DemigArgs a)
 {
@@ -1517,7 +1520,7 @@ k_demigrate(const DemigArgs a)
             for (int k = 0; k < K; ++k) {
                 const float qc = fast_clamp(fast_qs(dIS2, dIR2, rs[k], tq[k]), nsh);
                 int T = fast_T(qc);
-                if (fast_ambiguous(qc, KTM_FAST_EPS_S)) {   /* never past the trace */
+                if (!FASTIDX && fast_ambiguous(qc, KTM_FAST_EPS_S)) {   /* never past the trace */
                     const int t = slot_t(k);
                     const float v2 = a.pv2[pixc + t];
                     T = exact_T(dIS2, dIR2, a.t2[t], v2, 1.0f / v2, a.dt, a.rdt, ns);
@@ -2426,8 +2429,24 @@ static int demig_order(const ktm_plan *cp, const ktm_trace_geom *geom, int64_t n
     return KTM_OK;
 }
 
+static int demigrate(const ktm_plan *p, const float *km, const ktm_trace_geom *geom, int64_t ntr,
+                     float *traces, bool fast, void *stream);
+
 extern "C" int ktm_demigrate(const ktm_plan *p, const float *km, const ktm_trace_geom *geom,
                              int64_t ntr, float *traces, void *stream)
+{
+    return demigrate(p, km, geom, ntr, traces, false, stream);
+}
+
+extern "C" int ktm_demigrate_index(const ktm_plan *p, const float *km, const ktm_trace_geom *geom,
+                                   int64_t ntr, float *traces, int index, void *stream)
+{
+    if (index != 0 && index != KTM_INDEX_FAST) return KTM_ERR_ARG;
+    return demigrate(p, km, geom, ntr, traces, index == KTM_INDEX_FAST, stream);
+}
+
+static int demigrate(const ktm_plan *p, const float *km, const ktm_trace_geom *geom, int64_t ntr,
+                     float *traces, bool fast, void *stream)
 {
     if (!p || ntr < 0) return KTM_ERR_ARG;
     if (!ntr) return KTM_OK;
@@ -2468,13 +2487,13 @@ extern "C" int ktm_demigrate(const ktm_plan *p, const float *km, const ktm_trace
     if (a.tb < 1) return KTM_ERR_UNSUPPORTED;
     typedef void (*fn_t)(const DemigArgs);
     fn_t fn;
-    if (s->ns <= 512) fn = k_demigrate<8, 1>;
-    else if (s->ns <= 1024) fn = k_demigrate<8, 2>;
-    else if (s->ns <= 1536) fn = k_demigrate<12, 2>;
-    else if (s->ns <= 2048) fn = k_demigrate<8, 4>;
-    else if (s->ns <= 3072) fn = k_demigrate<12, 4>;
-    else if (s->ns <= 4096) fn = k_demigrate<8, 8>;
-    else fn = k_demigrate<12, 8>;
+    if (s->ns <= 512) fn = fast ? k_demigrate<8, 1, true> : k_demigrate<8, 1, false>;
+    else if (s->ns <= 1024) fn = fast ? k_demigrate<8, 2, true> : k_demigrate<8, 2, false>;
+    else if (s->ns <= 1536) fn = fast ? k_demigrate<12, 2, true> : k_demigrate<12, 2, false>;
+    else if (s->ns <= 2048) fn = fast ? k_demigrate<8, 4, true> : k_demigrate<8, 4, false>;
+    else if (s->ns <= 3072) fn = fast ? k_demigrate<12, 4, true> : k_demigrate<12, 4, false>;
+    else if (s->ns <= 4096) fn = fast ? k_demigrate<8, 8, true> : k_demigrate<8, 8, false>;
+    else fn = fast ? k_demigrate<12, 8, true> : k_demigrate<12, 8, false>;
     const size_t lds = row * (size_t)a.tb;
     HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     for (int64_t i0 = 0; i0 < ntr; i0 += 1 << 30) {

@@ -194,23 +194,26 @@ class Plan:
         return out
 
     # -- adjoint pair (CLOSED prep) ------------------------------------------
-    def migrate_lin(self, traces, coords, *, kahan: bool = False):
+    def migrate_lin(self, traces, coords, *, kahan: bool = False, fast: bool = False):
         """km = L^T d: the linear migration operator (CLOSED filter) on a zeroed
-        image; returns (km, fold)."""
+        image; returns (km, fold).  ``fast`` takes the relaxed traveltime
+        index (INDEX_FAST) instead of the reference's."""
         from ._lib import PREP_CLOSED
         prepared, geom = self.prep(traces, coords, PREP_CLOSED, TRACE_F32)
         km, fold, comp = self.new_image(kahan=kahan, device=traces.device)
-        self.migrate_block(prepared, geom, km, fold, comp, mode=PREP_CLOSED,
-                           accum=ACCUM_KAHAN if kahan else ACCUM_PLAIN)
+        accum = (ACCUM_KAHAN if kahan else ACCUM_PLAIN) | (INDEX_FAST if fast else 0)
+        self.migrate_block(prepared, geom, km, fold, comp, mode=PREP_CLOSED, accum=accum)
         return km, fold
 
-    def demigrate(self, km, geom):
-        """d = L km: the exact transpose of ``migrate_lin`` (modelling)."""
+    def demigrate(self, km, geom, *, fast: bool = False):
+        """d = L km: the exact transpose of ``migrate_lin`` with the same
+        ``fast`` (modelling)."""
         import torch
         ntr = geom.shape[0]
         out = torch.empty((ntr, self.ns), dtype=torch.float32, device=km.device)
-        check(lib().ktm_demigrate(self._h, _ptr(km.contiguous()), _ptr(geom), ntr, _ptr(out),
-                                  _stream()), "ktm_demigrate")
+        check(lib().ktm_demigrate_index(self._h, _ptr(km.contiguous()), _ptr(geom), ntr,
+                                        _ptr(out), INDEX_FAST if fast else 0, _stream()),
+              "ktm_demigrate_index")
         return out
 
     def poststack_adjoint(self, image, fold):

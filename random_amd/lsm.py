@@ -24,10 +24,11 @@ class KirchhoffMigration(torch.autograd.Function):
     backward applies L (and the post-stack transpose)."""
 
     @staticmethod
-    def forward(ctx, traces, plan: Plan, coords, poststack: bool = False):
-        km, fold = plan.migrate_lin(traces.contiguous(), coords, kahan=True)
+    def forward(ctx, traces, plan: Plan, coords, poststack: bool = False, fast: bool = False):
+        km, fold = plan.migrate_lin(traces.contiguous(), coords, kahan=True, fast=fast)
         geom = plan.geometry(coords)
         ctx.plan, ctx.geom, ctx.fold, ctx.poststack = plan, geom, fold, poststack
+        ctx.fast = fast
         if poststack:
             return plan.poststack(km, fold)
         return km
@@ -37,30 +38,32 @@ class KirchhoffMigration(torch.autograd.Function):
         g = grad_out.contiguous()
         if ctx.poststack:
             g = ctx.plan.poststack_adjoint(g, ctx.fold)
-        return ctx.plan.demigrate(g, ctx.geom), None, None, None
+        return ctx.plan.demigrate(g, ctx.geom, fast=ctx.fast), None, None, None, None
 
 
-def migrate_op(traces, plan: Plan, coords, poststack: bool = False):
-    return KirchhoffMigration.apply(traces, plan, coords, poststack)
+def migrate_op(traces, plan: Plan, coords, poststack: bool = False, fast: bool = False):
+    return KirchhoffMigration.apply(traces, plan, coords, poststack, fast)
 
 
 class Operator:
     """L / L^T on one survey with the geometry cached.  L^T accumulates with
     the Kahan-compensated FP32 kernel by default: at ~2e4 traces per image
     sample a plain FP32 sum carries ~1e-5 relative error, the size of the
-    adjointness tolerance."""
+    adjointness tolerance.  ``fast`` selects the pair with the relaxed
+    traveltime index (INDEX_FAST) on both sides."""
 
-    def __init__(self, plan: Plan, coords: torch.Tensor, kahan: bool = True):
+    def __init__(self, plan: Plan, coords: torch.Tensor, kahan: bool = True, fast: bool = False):
         self.plan = plan
         self.coords = coords
         self.kahan = kahan
+        self.fast = fast
         self.geom = plan.geometry(coords)
 
     def adjoint(self, d):      # L^T: traces -> image
-        return self.plan.migrate_lin(d, self.coords, kahan=self.kahan)[0]
+        return self.plan.migrate_lin(d, self.coords, kahan=self.kahan, fast=self.fast)[0]
 
     def forward(self, m):      # L: image -> traces
-        return self.plan.demigrate(m, self.geom)
+        return self.plan.demigrate(m, self.geom, fast=self.fast)
 
 
 def dot_test(op: Operator, d: torch.Tensor, m: torch.Tensor) -> tuple[float, float, float]:
