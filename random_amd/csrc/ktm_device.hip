@@ -628,6 +628,12 @@ __device__ __forceinline__ float clamp_ns(float q, float hi)
     return __builtin_amdgcn_fmed3f(q, 0.0f, hi);
 }
 
+/* Slots whose traveltime chains the exact path evaluates side by side
+ * under one test for the rare path (K divisible by it, else 2) */
+#ifndef KTM_SLOT_GROUP
+#define KTM_SLOT_GROUP 2
+#endif
+
 /* Queue of the speculative exact path: items per lane and staged batch (a
  * lane's further ambiguous samples of the batch take the inline exact path) */
 #define KTM_SPEC_Q 1
@@ -897,17 +903,67 @@ k_migrate(const MigArgs a)
                     }
                     return Fr[T];
                 };
+                /* the rare path of tap() for slot k: ambiguous or past the trace */
+                auto rare = [&](int k, int j, float q, float dIS2, float dIR2, int &T) __attribute__((always_inline)) {
+                    constexpr float EPS = KTM_FAST_EPS_S;
+                    if (q - nsf >= __builtin_fmaf(q, EPS, EPS)) {
+                        T = ns;   /* Q >= ns as well: past the trace */
+                        cnt[FASTIDX ? 0 : k] += 1;
+                    } else if (SPEC && qn < KTM_SPEC_Q) {
+                        sQw[qn * KTM_WAVE + lane] = make_uint2(
+                            __float_as_uint(KAHAN ? y[KAHAN ? k : 0] : acc[k]),
+                            (uint32_t)(k | (j << 4) | (T << 8)));
+                        ++qn;
+                    } else {
+                        const int tl = seg * K * KTM_WAVE + k * KTM_WAVE + lane;
+                        const float v2 = sV2c[tl];
+                        T = exact_T(dIS2, dIR2, a.t2[tl], v2, 1.0f / v2, a.dt, a.rdt, ns);
+                        cnt[FASTIDX ? 0 : k] += T >= ns ? 1 : 0;
+                    }
+                };
                 while (pm) {
                     const int j = __builtin_ctzll(pm);   /* traces in order */
                     pm &= pm - 1ull;
                     const float dIS2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g_is), j));
                     const float dIR2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g_ir), j));
                     const float *Fr = sF + j * ldl;
+                    constexpr int G = K % KTM_SLOT_GROUP == 0 ? KTM_SLOT_GROUP : 2;
+                    if (FASTIDX || (K % G)) {
 #pragma unroll
-                    for (int k = 0; k < K; ++k) {
-                        const float f = tap(k, j, dIS2, dIR2, Fr);
-                        if (KAHAN) y[KAHAN ? k : 0] = y[KAHAN ? k : 0] + f;
-                        else acc[k] = acc[k] + f;
+                        for (int k = 0; k < K; ++k) {
+                            const float f = tap(k, j, dIS2, dIR2, Fr);
+                            if (KAHAN) y[KAHAN ? k : 0] = y[KAHAN ? k : 0] + f;
+                            else acc[k] = acc[k] + f;
+                        }
+                    } else {
+                        /* exact index, G slots per branch: their chains run
+                         * side by side, one test guards their rare paths */
+#pragma unroll
+                        for (int k = 0; k < K; k += G) {
+                            constexpr float EPS = KTM_FAST_EPS_S;
+                            float qg[G];
+                            int Tg[G];
+                            bool ag[G], any = false;
+#pragma unroll
+                            for (int i = 0; i < G; ++i) {
+                                qg[i] = fast_qs(dIS2, dIR2, rs[k + i], tq[k + i]);
+                                const float qc = clamp_ns(qg[i], nsf);
+                                Tg[i] = fast_T(qc);
+                                ag[i] = fast_ambiguous(qc, EPS);
+                                any = any || ag[i];
+                            }
+                            if (any) {
+#pragma unroll
+                                for (int i = 0; i < G; ++i)
+                                    if (ag[i]) rare(k + i, j, qg[i], dIS2, dIR2, Tg[i]);
+                            }
+#pragma unroll
+                            for (int i = 0; i < G; ++i) {
+                                const float f = Fr[Tg[i]];
+                                if (KAHAN) y[KAHAN ? k + i : 0] = y[KAHAN ? k + i : 0] + f;
+                                else acc[k + i] = acc[k + i] + f;
+                            }
+                        }
                     }
                 }
                 if (SPEC && __ballot(qn > 0)) {
