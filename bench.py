@@ -34,10 +34,10 @@ BYTES_FP32 = 12.0   # three FP32 trace taps per contribution
 BYTES_HALF = 6.0    # three 16-bit taps
 METRIC = "G trace×image-pt contributions/s, %HBM roofline; BF16-trace vs FP32 at err"
 # HBM bytes per launch of k_migrate on this workload, from rocprofv3 --pmc
-# FETCH_SIZE (437,011,974 KiB, scaled x2.0) + WRITE_SIZE (1,881,561 KiB, x1.0),
+# FETCH_SIZE (465,868,004 KiB, scaled x2.0) + WRITE_SIZE (1,882,132 KiB, x1.0),
 # the scale factors being what the same counters read for a 1 GiB device copy
 # (profiles/k_migrate_area3d_1e6.md)
-TRAFFIC = 896_927_240_448
+TRAFFIC = 956_024_975_360
 
 
 def log(*a):
@@ -176,11 +176,11 @@ def valu_probe(dev, modes=tuple(VALU_MODES), waves_per_simd=(8,), target_ms=40.0
 # v_min/v_med3, v_rndne, v_cmp, v_cndmask, v_readlane are in no class.
 KMIG_COUNTS = {
     "exact": {"kernel": "k_migrate<12, 2, 8, false, false>", "contributions": 1806379368000,
-              "launch_ms": 958.56, "clock_ghz": 2.382, "valu": 488628380575,
+              "launch_ms": 942.06, "clock_ghz": 2.381, "valu": 488628380575,
               "fma_f32": 94969484010, "add_f32": 97736566828, "mul_f32": 5946021325,
               "trans_f32": 58496827853, "cvt": 30618638965, "int32": 61308780164,
               "fma_f64": 5207598064, "add_f64": 370325576, "mul_f64": 1532187535,
-              "trans_f64": 744491152, "int64": 6386285760, "salu": 132365233677,
+              "trans_f64": 744491152, "int64": 6386285760, "salu": 128120225353,
               "lds": 36591172744},
     "fast_kahan": {"kernel": "k_migrate<12, 2, 8, true, true>", "contributions": 1806379368000,
                    "launch_ms": 748.03, "clock_ghz": 2.377, "valu": 371632398974,
