@@ -64,7 +64,7 @@ def parse():
                     help="traces of the FP64-oracle error check on the 50x50x1500 sub-volume")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy 3x (HBM counter calibration) and exit")
-    ap.add_argument("--leg", choices=("vgrad", "adjoint", "poststack", "eom", "valu"),
+    ap.add_argument("--leg", choices=("vgrad", "adjoint", "poststack", "eom", "valu", "ingest"),
                     help="only run this extra leg, print its JSON and exit (for profiling)")
     return ap.parse_args()
 
@@ -493,7 +493,100 @@ def ingest_leg(sv, plan, traces, block, dev, reps=5):
         out[k] = {"ms": round(ms, 3), "bytes": int(b), "GB_per_s": round(gbs, 1),
                   "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
         log(f"ingest {k}: {ms:.3f} ms, {gbs:.0f} GB/s")
+    out["file"] = file_leg(plan, raw, hdr, n, dev)
     return out
+
+
+def ingest_only_leg(dev, n=50_000):
+    """The ingest leg alone on the first block of the 3-D survey."""
+    import numpy as np
+
+    from random_amd import surveys, synth
+    from random_amd.ktm import Plan
+    sv = surveys.area3d()
+    sv.coords = np.ascontiguousarray(sv.coords[:n])
+    plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
+    traces, _ = synth.traces_gpu(plan, sv, device=dev)
+    return ingest_leg(sv, plan, traces, n, dev)
+
+
+def file_leg(plan, raw, hdr, n, dev):
+    """File I/O of the same block, reported apart from the in-memory figures:
+    the records written as an IBM SEG-Y file (3600-byte file header, ns and
+    dt in every trace header), then read back three ways: the host reader
+    (ktm_segy_read: fread + IBM decode on one core, as Kirchhoff.c:1281-1290
+    does), the raw records into pinned memory, and those copied to the device
+    and ingested (ktm_ingest).  The file was just written, so the reads come
+    from the page cache, not the disk."""
+    import os
+    import shutil
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    from random_amd import _lib
+    from random_amd._lib import PREP_REFERENCE, SEGY_IBM, TRACE_F32
+    ns = plan.ns
+    rec = 240 + 4 * ns
+    host = raw.cpu().numpy().reshape(n, rec).copy()
+    host[:, 114:116] = np.array([ns], dtype=">u2").view(np.uint8)
+    host[:, 116:118] = np.array([round(plan.dt * 1e6)], dtype=">u2").view(np.uint8)
+    d = tempfile.mkdtemp(prefix="ktm_io_")
+    path = os.path.join(d, "block.sgy")
+    try:
+        t0 = time.perf_counter()
+        with open(path, "wb") as f:
+            f.write(bytes(3600))
+            f.write(host.tobytes())
+            f.flush()
+            os.fsync(f.fileno())
+        t_write = time.perf_counter() - t0
+        nbytes = 3600 + host.nbytes
+        samples = np.empty((n, ns), np.float32)
+        coords = np.empty((n, 4), np.int32)
+        L = _lib.lib()
+        t0 = time.perf_counter()
+        _lib.check(L.ktm_segy_read(path.encode(), 0, n, 0, samples.ctypes.data, coords.ctypes.data,
+                                   None), "ktm_segy_read")
+        t_host = time.perf_counter() - t0
+        pinned = torch.empty(host.nbytes, dtype=torch.uint8, pin_memory=True)
+        view = pinned.numpy()
+        t0 = time.perf_counter()
+        with open(path, "rb") as f:
+            f.seek(3600)
+            got = f.readinto(memoryview(view))
+        t_read = time.perf_counter() - t0
+        assert got == host.nbytes
+        d_raw = torch.empty(host.nbytes, dtype=torch.uint8, device=dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        d_raw.copy_(pinned, non_blocking=True)
+        torch.cuda.synchronize()
+        t_h2d = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        prepared, geom, dco = plan.ingest(d_raw, n, SEGY_IBM, PREP_REFERENCE, TRACE_F32)
+        torch.cuda.synchronize()
+        t_ing = time.perf_counter() - t0
+        # the coordinates through the file: host reader, device ingest, headers written
+        want = host[:, 72:88].copy().view(">i4").astype(np.int32)
+        ok = bool(np.array_equal(coords, want) and np.array_equal(dco.cpu().numpy(), want))
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    gb = nbytes / 1e9
+    res = {"file_bytes": int(nbytes),
+           "write_s": round(t_write, 3), "write_GB_per_s": round(gb / t_write, 2),
+           "host_read_decode_s": round(t_host, 3),
+           "host_read_decode_GB_per_s": round(gb / t_host, 2),
+           "raw_read_pinned_s": round(t_read, 3), "raw_read_GB_per_s": round(gb / t_read, 2),
+           "h2d_s": round(t_h2d, 4), "h2d_GB_per_s": round(host.nbytes / 1e9 / t_h2d, 1),
+           "device_ingest_s": round(t_ing, 4),
+           "file_to_prepared_on_device_s": round(t_read + t_h2d + t_ing, 3),
+           "coords_round_trip_equal": ok,
+           "note": "page-cache reads of a just-written file (fsync'd), one host thread"}
+    log(f"file: {gb:.2f} GB written {t_write:.2f} s; host read+decode {t_host:.2f} s; raw read "
+        f"{t_read:.3f} s, H2D {t_h2d * 1e3:.1f} ms, ingest {t_ing * 1e3:.1f} ms")
+    return res
 
 
 def vgrad_leg(dev, reps=3, fd_nodes=10):
@@ -770,7 +863,7 @@ def main():
     if args.leg:
         import torch
         fn = {"vgrad": vgrad_leg, "adjoint": adjoint_leg, "poststack": poststack_leg,
-              "eom": eom_leg,
+              "eom": eom_leg, "ingest": ingest_only_leg,
               "valu": lambda d: valu_probe(d, waves_per_simd=(1, 2, 4, 8))}[args.leg]
         print(json.dumps(fn(torch.device("cuda:0"))), flush=True)
         return
