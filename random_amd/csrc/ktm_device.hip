@@ -1643,6 +1643,33 @@ __global__ void __launch_bounds__(PS_NT) k_poststack_adjoint_tail(const float *_
 
 /* ------------------------------------------------------------------------ */
 /* host glue                                                                 */
+/* The velocity check of ktm_plan_create on the device: every sample an image
+ * column reads (column x, y at grid node velx[x], vely[y], time index t1[t])
+ * must be > 0 and < 3e19, whose square stays finite (a velocity <= 0 or NaN
+ * would divide by zero at Kirchhoff.c:1381); a failing sample sets *bad. */
+__global__ void k_velocity_check(const float *__restrict__ vel, const int32_t *__restrict__ velx,
+                                 const int32_t *__restrict__ vely, const int32_t *__restrict__ t1,
+                                 int nxp, int nyp, int ns, int nvel_xl, int nsm, uint32_t *bad)
+{
+    const int64_t n = (int64_t)nxp * nyp * ns;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int t = (int)(e % ns);
+        const int64_t c = e / ns;
+        const int x = (int)(c / nyp), y = (int)(c % nyp);
+        const float v = vel[((int64_t)velx[x] * nvel_xl + vely[y]) * nsm + t1[t]];
+        if (!(v > 0.0f) || !(v < 3.0e19f)) *bad = 1u;
+    }
+}
+
+/* v -> v^2 in place (Kirchhoff.c:617), one FP32 product each */
+__global__ void k_square(float *__restrict__ v, int64_t n)
+{
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x)
+        v[e] = v[e] * v[e];
+}
+
 /* ------------------------------------------------------------------------ */
 template <typename T>
 static int upload(T **dst, const T *src, size_t n)
@@ -1684,24 +1711,8 @@ extern "C" int ktm_plan_create(ktm_plan **out, const ktm_config *cfg, int ns, fl
         return st;
     }
     p->row = ((int64_t)ns + 1 + 7) / 8 * 8;
-    /* square the velocity model (Kirchhoff.c:617) and check the part the
-     * image uses: a velocity <= 0 (or NaN) would divide by zero at 1381 */
     const ktm_tables *tb = &p->tb;
-    const size_t nv = (size_t)cfg->nvel_ilines * cfg->nvel_xlines * nsm;
-    float *v2 = (float *)malloc(sizeof(float) * nv);
-    if (!v2) { ktm_plan_destroy(p); return KTM_ERR_NOMEM; }
-    for (size_t k = 0; k < nv; ++k) v2[k] = vel[k] * vel[k];
-    int bad = 0;
-    for (int x = 0; x < tb->s.nxp && !bad; ++x)
-        for (int y = 0; y < tb->s.nyp && !bad; ++y) {
-            const float *col = vel + ((size_t)tb->velx[x] * cfg->nvel_xlines + tb->vely[y]) * nsm;
-            for (int t = 0; t < ns; ++t)
-                if (!(col[tb->t1[t]] > 0.0f) || !(col[tb->t1[t]] < 3.0e19f)) { bad = 1; break; }
-        }
-    if (bad) { free(v2); ktm_plan_destroy(p); return KTM_ERR_VELOCITY; }
-    st = upload(&p->d_vel2, v2, nv);
-    free(v2);
-    if (st == KTM_OK) st = upload(&p->d_t2, tb->t2, (size_t)ns);
+    st = upload(&p->d_t2, tb->t2, (size_t)ns);
     if (st == KTM_OK) st = upload(&p->d_t1, tb->t1, (size_t)ns);
     if (st == KTM_OK) st = upload(&p->d_colx, tb->colx, (size_t)tb->s.nxp);
     if (st == KTM_OK) st = upload(&p->d_coly, tb->coly, (size_t)tb->s.nyp);
@@ -1710,6 +1721,29 @@ extern "C" int ktm_plan_create(ktm_plan **out, const ktm_config *cfg, int ns, fl
     if (st == KTM_OK) st = upload(&p->d_slab_cx, tb->slab_cx, (size_t)tb->s.no_job);
     if (st == KTM_OK) st = upload(&p->d_slab_cy, tb->slab_cy, (size_t)tb->s.no_job);
     if (st == KTM_OK) st = upload(&p->d_slab_x2thr, tb->slab_x2thr, (size_t)tb->s.no_job);
+    /* the velocity model: checked where the image reads it, then squared
+     * (Kirchhoff.c:617), on the device */
+    const size_t nv = (size_t)cfg->nvel_ilines * cfg->nvel_xlines * nsm;
+    if (st == KTM_OK) st = upload(&p->d_vel2, vel, nv);
+    if (st == KTM_OK) {
+        uint32_t *d_bad = nullptr, bad = 0;
+        if (hipMalloc((void **)&d_bad, sizeof(uint32_t)) != hipSuccess) {
+            st = KTM_ERR_NOMEM;
+        } else {
+            const int64_t nchk = (int64_t)tb->s.nxp * tb->s.nyp * ns;
+            const unsigned g = (unsigned)(nchk / 256 < 8192 ? (nchk + 255) / 256 : 8192);
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, 0, d_bad);
+            hipLaunchKernelGGL(k_velocity_check, dim3(g), dim3(256), 0, 0, p->d_vel2, p->d_velx,
+                               p->d_vely, p->d_t1, tb->s.nxp, tb->s.nyp, ns, cfg->nvel_xlines, nsm,
+                               d_bad);
+            hipLaunchKernelGGL(k_square, dim3(8192), dim3(256), 0, 0, p->d_vel2, (int64_t)nv);
+            if (hipGetLastError() != hipSuccess ||
+                hipMemcpy(&bad, d_bad, sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
+                st = KTM_ERR_DEVICE;
+            (void)hipFree(d_bad);
+            if (st == KTM_OK && bad) st = KTM_ERR_VELOCITY;
+        }
+    }
     if (st == KTM_OK) st = upload(&p->d_w, tb->w, (size_t)ns + 2);
     if (st == KTM_OK) {
         double *h2 = (double *)malloc(sizeof(double) * 2 * (size_t)ns);
