@@ -765,11 +765,10 @@ def poststack_leg(dev, plan=None, reps=3):
         ms_r.append(e1.elapsed_time(e2))
     ms_p, ms_r = float(np.median(ms_p)), float(np.median(ms_r))
     ntraces = int(np.prod(plan.image_shape[:3]))
-    # the FFT form: two traces per complex transform of N = 2^k >= 2 ns - 1
-    # points, forward and inverse, 5 N log2 N flop each (the radix-2 count)
-    nfft = 256
-    while nfft < 2 * ns - 1:
-        nfft *= 2
+    # the FFT form: two traces per complex transform of N >= 2 ns - 1 points
+    # (256 ... 8192 and 3072), forward and inverse, 5 N log2 N flop each (the
+    # radix-2 count)
+    nfft = next(n for n in (256, 512, 1024, 2048, 3072, 4096, 8192) if n >= 2 * ns - 1)
     flop = (ntraces / 2.0) * 2 * 5.0 * nfft * np.log2(nfft)
     bytes_p = 3 * 4 * ntraces * ns            # km and fold in, image out
     bytes_r = 4 * nrec * ns + rec.numel()     # image in, records out

@@ -65,7 +65,7 @@ struct ktm_plan {
     float *d_w;             /* [ns+2] CLOSED weights */
     double *d_hconv2;       /* [2 ns] post-stack kernel, h2[m] = h[m mod ns] */
     double2 *d_pf_g, *d_pf_tw; /* [N] FFT form of the post-stack kernel and twiddles */
-    int pf_logn;            /* log2 N of the FFT form, 0: the direct product */
+    int pf_n;                /* points of the FFT form, 0: the direct product */
     float *d_stage;         /* FP32 filtered traces of a BF16/FP16 block */
     size_t stage_cap;       /* elements of d_stage */
     float *d_pix_v2, *d_pix_rs; /* per-pixel v^2 and RN(1/(v^2 dt^2)), built for demigration */
@@ -1752,12 +1752,12 @@ extern "C" int ktm_plan_create(ktm_plan **out, const ktm_config *cfg, int ns, fl
         if (st == KTM_OK) st = upload(&p->d_hconv2, h2, 2 * (size_t)ns);
         free(h2);
     }
-    if (st == KTM_OK && (p->pf_logn = postfft_logn(ns)) != 0) {
-        const size_t n = (size_t)1 << p->pf_logn;
+    if (st == KTM_OK && (p->pf_n = postfft_size(ns)) != 0) {
+        const size_t n = (size_t)p->pf_n;
         double *g = (double *)malloc(sizeof(double) * 4 * n);
         if (!g) st = KTM_ERR_NOMEM;
         if (st == KTM_OK) {
-            postfft_host_tables(tb->hconv, ns, p->pf_logn, g, g + 2 * n);
+            postfft_host_tables(tb->hconv, ns, p->pf_n, g, g + 2 * n);
             st = upload(&p->d_pf_g, reinterpret_cast<const double2 *>(g), n);
             if (st == KTM_OK) st = upload(&p->d_pf_tw, reinterpret_cast<const double2 *>(g + 2 * n), n);
         }
@@ -2424,7 +2424,7 @@ extern "C" int ktm_poststack(const ktm_plan *p, const float *km, const float *fo
     const ktm_setup *s = &p->tb.s;
     const int64_t ntraces = (int64_t)s->nxp * s->nyp * s->nh;
     const hipError_t e =
-        p->pf_logn ? postfft_launch<PostKtm>(km, fold, ntraces, s->ns, s->dt, p->d_pf_g, p->d_pf_tw,
+        p->pf_n ? postfft_launch<PostKtm>(km, fold, ntraces, s->ns, s->dt, p->d_pf_g, p->d_pf_tw,
                                              out, (hipStream_t)stream)
                    : postmx_launch<PostKtm>(km, fold, ntraces, s->ns, s->dt, p->d_hconv2, out,
                                             (hipStream_t)stream);
@@ -2609,7 +2609,7 @@ extern "C" int ktm_poststack_adjoint(const ktm_plan *p, const float *img, const 
     if (ntraces > 0x7fffffffLL) return KTM_ERR_UNSUPPORTED;
     hipStream_t hs = (hipStream_t)stream;
     const hipError_t e =
-        p->pf_logn ? postfft_launch<PostImg>(img, fold, ntraces, s->ns, s->dt, p->d_pf_g,
+        p->pf_n ? postfft_launch<PostImg>(img, fold, ntraces, s->ns, s->dt, p->d_pf_g,
                                              p->d_pf_tw, km_bar, hs)
                    : postmx_launch<PostImg>(img, fold, ntraces, s->ns, s->dt, p->d_hconv2, km_bar,
                                             hs);

@@ -145,7 +145,7 @@ struct ktm_eom_plan {
     double *d_sv;             /* [ns] sqrt(v^2) in double (EOM.c:1290) */
     double *d_g2;             /* [2 ns] post-filter kernel, g2[m] = g[m mod ns] */
     double2 *d_pf_g, *d_pf_tw; /* [N] FFT form of the post-filter kernel and twiddles */
-    int pf_logn;              /* log2 N of the FFT form, 0: the direct product */
+    int pf_n;                  /* points of the FFT form, 0: the direct product */
     float4 *d_geo;            /* [cap] per-trace h2, cmpX, cmpY, talpha */
     int64_t geo_cap;
     float v2_0;               /* v^2 at t = 0 (talpha, EOM.c:1272) */
@@ -229,12 +229,12 @@ extern "C" int ktm_eom_plan_create(ktm_eom_plan **out, const ktm_eom_config *c, 
          hipMemcpy(p->d_sv, sv, sizeof(double) * ns, hipMemcpyHostToDevice) != hipSuccess ||
          hipMemcpy(p->d_g2, g, sizeof(double) * 2 * ns, hipMemcpyHostToDevice) != hipSuccess))
         st = KTM_ERR_DEVICE;
-    if (st == KTM_OK && (p->pf_logn = postfft_logn(ns)) != 0) {
-        const size_t n = (size_t)1 << p->pf_logn;
+    if (st == KTM_OK && (p->pf_n = postfft_size(ns)) != 0) {
+        const size_t n = (size_t)p->pf_n;
         double *t = (double *)malloc(sizeof(double) * 4 * n);
         if (!t) st = KTM_ERR_NOMEM;
         if (st == KTM_OK) {
-            postfft_host_tables(g, ns, p->pf_logn, t, t + 2 * n);
+            postfft_host_tables(g, ns, p->pf_n, t, t + 2 * n);
             if (hipMalloc(&p->d_pf_g, sizeof(double2) * n) != hipSuccess ||
                 hipMalloc(&p->d_pf_tw, sizeof(double2) * n) != hipSuccess)
                 st = KTM_ERR_NOMEM;
@@ -554,7 +554,7 @@ static int eom_post(const ktm_eom_plan *p, const float *csp, const float *fold, 
     if (ns > 8192) return KTM_ERR_UNSUPPORTED;
     EOM_TRY(hipSetDevice(p->device));
     const hipError_t e =
-        p->pf_logn ? postfft_launch<PostEom>(csp, fold, ntraces, ns, p->s.dt, p->d_pf_g, p->d_pf_tw,
+        p->pf_n ? postfft_launch<PostEom>(csp, fold, ntraces, ns, p->s.dt, p->d_pf_g, p->d_pf_tw,
                                              out, (hipStream_t)stream)
                    : postmx_launch<PostEom>(csp, fold, ntraces, ns, p->s.dt, p->d_g2, out,
                                             (hipStream_t)stream);

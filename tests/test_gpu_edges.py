@@ -140,14 +140,16 @@ def test_plan_size_limits(cuda):
         assert e.value.status == _lib.KTM_ERR_UNSUPPORTED
 
 
-@pytest.mark.parametrize("ns", [4, 128, 129, 1025, 2048, 2049, 4096, 4097, 6144])
+@pytest.mark.parametrize("ns", [4, 128, 129, 512, 1024, 1025, 1536, 1537, 2048, 2049, 4096, 4097,
+                                6144])
 def test_poststack_long_traces(cuda, ns):
     """Forward against the oracle's DFT path and the dot test of the
     transpose on both sides of every transform length of the FFT form
-    (N = 256 up to 128 samples, 512 from 129, ..., 4096 up to 2048, 8192 from
-    2049 to 4096) and of the switch to the FP64 matrix product beyond 4096
-    samples; 8 image traces at even ns, 7 at odd ns (the last transform then
-    carries one trace)."""
+    (N = 256 up to 128 samples, 512 from 129, 1024, 2048, 3072 from 1025 to
+    1536 (radix 16 x 16 x 12), 4096 from 1537 to 2048, 8192 from 2049 to
+    4096) and of the switch to the FP64 matrix product beyond 4096 samples;
+    8 image traces at even ns, 7 at odd ns (the last transform then carries
+    one trace)."""
     print(f"case: post-stack and transpose at ns={ns}", flush=True)
     sv = small_line(ns, ncols=7 if ns % 2 else 8, jobs_x=1)
     plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
