@@ -243,3 +243,39 @@ def test_compensated_line_vs_fp64(cuda, nh, jobs_x, block):
     assert n == n64
     assert np.array_equal(fold.cpu().numpy(), fold64.astype(np.float32))
     assert r <= 1e-6
+
+
+def test_compensated_split_per_job_region(cuda):
+    """The compensated split on job regions: a 3-D survey whose 2 x 3 job
+    slabs are migrated one at a time (job >= 0 restricts a launch to the
+    slab's columns, Kirchhoff.c:1341-1347), so each launch splits its traces
+    into parts over a region smaller than the image and merges partial
+    images laid out by the region.  The union of the slabs equals the
+    oracle's FP64 image to 1e-6 and its fold and contribution count exactly,
+    and the columns outside a slab stay untouched."""
+    sv = surveys.area3d(ntr=3000, ns=700, ncols=24, extent=800.0, max_offset=2000.0, nscat=4,
+                        jobs_x=2, jobs_y=3)
+    tr = surveys.traces_numpy(sv)
+    km64, fold64, n64 = oracle.migrate(sv.cfg, sv.ns, sv.dt, tr, sv.coords, sv.vel,
+                                       mode=oracle.FP64_DATA)
+    plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
+    print(f"case: compensated split per job region, {sv.ntr} traces, 24 x 24 columns, 6 jobs",
+          flush=True)
+    d_tr, d_co = torch.from_numpy(tr).to(cuda), torch.from_numpy(sv.coords).to(cuda)
+    prepared, geom = plan.prep(d_tr, d_co, PREP_CLOSED, TRACE_F32)
+    km, fold, comp = plan.new_image(kahan=True, device=cuda)
+    counter = torch.zeros(1, dtype=torch.int64, device=cuda)
+    for job in range(6):
+        plan.migrate_block(prepared, geom, km, fold, comp, mode=PREP_CLOSED, accum=ACCUM_KAHAN,
+                           job=job, counter=counter)
+        if job == 0:
+            # only slab 0's columns (12 x 8 of the 24 x 24) have been written
+            f0 = fold.cpu().numpy()
+            assert np.all(f0[12:] == 1.0) and np.all(f0[:, 8:] == 1.0)
+            assert np.any(f0[:12, :8] > 1.0)
+    torch.cuda.synchronize()
+    r = rel_l2(km.cpu().numpy(), km64)
+    print(f"  relL2 {r:.3e}, contributions {int(counter.item())} (oracle {n64})", flush=True)
+    assert int(counter.item()) == n64
+    assert np.array_equal(fold.cpu().numpy(), fold64.astype(np.float32))
+    assert r <= 1e-6
