@@ -296,9 +296,12 @@ int ktm_vgrad_create(ktm_vgrad **out, const ktm_config *cfg, int ns, float dt, i
                      int nx_nodes, int chunk);
 int ktm_vgrad_destroy(ktm_vgrad *vg);
 /* Use a block of CLOSED-prepared traces (ktm_prep with KTM_PREP_CLOSED,
- * F32; row = ktm_plan_trace_row) and their geometry, both on the device and
- * kept alive by the caller.  Bins the traces by (offset class, CMP) on the
- * host; synchronises the stream. */
+ * F32; row = ktm_plan_trace_row) and their geometry, both on the device.
+ * Bins the traces by (offset class, CMP) on the host and copies them, in
+ * that order and zero-padded, into a table of (traces in classes) x (ns + 6)
+ * floats that the handle holds with the binned geometry (grown on a larger
+ * rebind); prepared and geom may be released on return.  Synchronises the
+ * stream. */
 int ktm_vgrad_bind(ktm_vgrad *vg, const float *prepared, int64_t row,
                    const ktm_trace_geom *geom, int64_t ntr, void *stream);
 /* v(t,X) of node velocities V[nt_nodes][nx_nodes] on the image grid

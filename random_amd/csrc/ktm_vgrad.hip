@@ -26,10 +26,14 @@
  * to accumulate
  *   dJ/dv(X,t) = sum_h dJ/dI[h][t] sum_c F'(q_c) dq_c/dv.
  * Nothing per contribution is stored: the time chunk and a window of three
- * classes bound the working set to registers.  Traces are binned on the host by (offset
- * class, CMP inline coordinate), so a column's traces of one class are a
- * contiguous range; every wave walks the ranges itself (gates lane-parallel,
- * ballot, then the passing traces in order) and needs no barrier.
+ * classes bound the working set to registers.  Traces are binned on the host
+ * by (offset class, CMP inline coordinate), so a column's traces of one class
+ * are a contiguous range; every wave walks the ranges itself (gates
+ * lane-parallel, ballot, then the passing traces in order) and needs no
+ * barrier.  At bind, k_vg_tables lays the geometry and the traces out in
+ * that order, each trace zero-padded (one sample before, five after): the
+ * four taps of a contribution are one 16-byte load, with no clamp and no
+ * trace-index lookup in the loop.
  * k_vg_project_t / k_vg_project_x: dJ/dV = B_t^T (dJ/dv) B_x in FP64.
  */
 #include <hip/hip_runtime.h>
@@ -43,11 +47,10 @@
 
 #define VG_NT 256
 #define VG_WAVE 64
-/* passing traces whose taps are loaded together: with the classes streamed
- * (no LDS) 2 keeps the kernel at 38 VGPRs and 8 waves per SIMD, which hide
- * the L2 latency of the taps; 1 to 4 measured the same, 8 (107 VGPRs) 12 %
- * and 16 (164 VGPRs, 3 waves) 23 % slower */
-#define VG_G 2
+/* passing traces evaluated per step, their tap loads issued together: 4
+ * (44 VGPRs, 8 waves per SIMD) measured 8 % faster than 1 or 2 and 2 %
+ * faster than 6 (60 VGPRs) or 8 (76 VGPRs, 6 waves) */
+#define VG_G 4
 
 struct ktm_vgrad {
     ktm_config cfg;
@@ -66,9 +69,9 @@ struct ktm_vgrad {
     int32_t *d_order;      /* [nbin] trace indices sorted by (class, cmpx) */
     int32_t *d_ranges;     /* [nxp][nh][2] */
     int64_t order_cap;
-    const float *prep;
-    int64_t row;
-    const ktm_trace_geom *geom;
+    ktm_trace_geom *d_sgeom;  /* [nbin] geometry in binned order */
+    float *d_taps;         /* [nbin][ns + 6] zero-padded traces in binned order */
+    int64_t taps_cap;      /* rows of d_taps and d_sgeom */
     int64_t ntr;
     int64_t pairs;         /* (trace, column) pairs passing the aperture gate */
     int bound;
@@ -149,10 +152,10 @@ __global__ void k_vg_project_x(const double *__restrict__ G1, int nxp, int nt_no
 /* migration into offset-class gathers, objective and dJ/dv                  */
 /* ------------------------------------------------------------------------ */
 struct VgArgs {
-    const float *prep;             /* [ntr][row] CLOSED filtered traces, zero for T >= ns */
-    int64_t row;
-    const ktm_trace_geom *geom;
-    const int32_t *order, *ranges;
+    const float *taps;             /* [nbin][trow] zero-padded CLOSED traces, binned order */
+    int64_t trow;                  /* ns + 6 */
+    const ktm_trace_geom *sgeom;   /* [nbin] geometry in binned order */
+    const int32_t *ranges;
     const float *v, *colx;
     float coly, ap_x2thr, dt, rdt;
     int ns, nh, nxp, chunk, nchunks, run;
@@ -161,17 +164,39 @@ struct VgArgs {
     double *part;                  /* [nxp * nchunks] */
 };
 
+/* four consecutive floats at a 4-byte-aligned address (one dwordx4 load) */
+typedef float vg_f4 __attribute__((ext_vector_type(4), aligned(4)));
+
 /* Catmull-Rom cubic through the taps f = F[T-1..T+2] at q = T + u, u in
- * [0, 1) (taps outside [0, ns) read the zero at F[ns]); returns the value,
- * or with DERIV its q-derivative */
+ * [0, 1) (taps outside [0, ns) are zero); returns the value, or with DERIV
+ * its q-derivative */
 template <bool DERIV>
-__device__ __forceinline__ float cr_eval(const float f[4], float u)
+__device__ __forceinline__ float cr_eval(const vg_f4 f, float u)
 {
-    const float c1 = 0.5f * (f[2] - f[0]);
-    const float c2 = __builtin_fmaf(-0.5f, f[3], __builtin_fmaf(2.0f, f[2], __builtin_fmaf(-2.5f, f[1], f[0])));
-    const float c3 = __builtin_fmaf(1.5f, f[1] - f[2], 0.5f * (f[3] - f[0]));
+    const float c1 = 0.5f * (f.z - f.x);
+    const float c2 = __builtin_fmaf(-0.5f, f.w, __builtin_fmaf(2.0f, f.z, __builtin_fmaf(-2.5f, f.y, f.x)));
+    const float c3 = __builtin_fmaf(1.5f, f.y - f.z, 0.5f * (f.w - f.x));
     if (DERIV) return __builtin_fmaf(u, __builtin_fmaf(3.0f * u, c3, 2.0f * c2), c1);
-    return __builtin_fmaf(u, __builtin_fmaf(u, __builtin_fmaf(u, c3, c2), c1), f[1]);
+    return __builtin_fmaf(u, __builtin_fmaf(u, __builtin_fmaf(u, c3, c2), c1), f.y);
+}
+
+/* The binned tables: sgeom[k] = geom[order[k]] and taps[k][p] = F[p - 1] of
+ * trace order[k] for 1 <= p <= ns, zero for p = 0 and ns < p < trow
+ * (trow = ns + 6), so the taps F[T-1..T+2] of every interval T in
+ * [0, ns + 2] are taps[k][T..T+3], read without a clamp.  Grid: x over p,
+ * y strides over the binned traces. */
+__global__ void k_vg_tables(const float *__restrict__ prep, int64_t row,
+                            const ktm_trace_geom *__restrict__ geom,
+                            const int32_t *__restrict__ order, int64_t nbin, int ns, int64_t trow,
+                            float *__restrict__ taps, ktm_trace_geom *__restrict__ sgeom)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= trow) return;
+    for (int64_t k = blockIdx.y; k < nbin; k += gridDim.y) {
+        const int32_t ti = order[k];
+        taps[k * trow + p] = (p >= 1 && p <= ns) ? prep[(int64_t)ti * row + p - 1] : 0.0f;
+        if (p == 0) sgeom[k] = geom[ti];
+    }
 }
 
 /* One pass over the traces of class h for this lane's sample t.  The
@@ -179,22 +204,22 @@ __device__ __forceinline__ float cr_eval(const float f[4], float u)
  *   m = a r / (s1 + th) + b r / (s2 + th),  s1 = sqrt(th^2 + a r),  th = t dt / 2,
  * (equal to (s1 + s2)/dt - t), so FP32 rounds the moveout and not the whole
  * traveltime: ulp(m) instead of ulp(q) (1e-4 samples at q ~ 2000).
- * FWD: returns the Kahan sum of F(q); otherwise sum F'(q) (a/s1 + b/s2). */
+ * FWD: returns the Kahan sum of F(q); otherwise sum F'(q) (a/s1 + b/s2).
+ * lo..hi are positions in the binned order, so they index sgeom and taps
+ * directly.  The interval T = t + floor(min(m/dt, ns + 2 - t)) stays in
+ * [0, ns + 2] for every lane (m >= 0; inactive lanes have r = 0). */
 template <bool FWD>
 __device__ __forceinline__ float vg_class_pass(const VgArgs &a, int lo, int hi, float IX,
                                                float r, float th, float t2, int t, int lane)
 {
-    const unsigned nsu = (unsigned)a.ns;
-    const float dmax = (float)(a.ns + 2 - t);   /* q clamped to ns + 2: every tap zero */
+    const float dmax = (float)(a.ns + 2 - t);   /* q clamped to ns + 2: a zero interval */
     float acc = 0.0f, cmp = 0.0f;
     for (int b0 = lo; b0 < hi; b0 += VG_WAVE) {
         const int k = b0 + lane;
         float ga = 0.0f, gb = 0.0f;
-        int ti = 0;
         bool pass = false;
         if (k < hi) {
-            ti = a.order[k];
-            const ktm_trace_geom g = a.geom[ti];
+            const ktm_trace_geom g = a.sgeom[k];
             const float ax = IX - g.sxt, ay = a.coly - g.syt;
             const float bx = IX - g.gxt, by = a.coly - g.gyt;
             const float cx = IX - g.cmpx, cy = a.coly - g.cmpy;
@@ -204,9 +229,8 @@ __device__ __forceinline__ float vg_class_pass(const VgArgs &a, int lo, int hi, 
         }
         uint64_t m = __ballot(pass);
         while (m) {
-            /* up to VG_G passing traces at a time, in order: their taps are
-             * loaded together so that the loads overlap; slots past the last
-             * passing trace repeat the first and are not added */
+            /* up to VG_G passing traces at a time, in order; slots past the
+             * last passing trace repeat the first and are not added */
             int jc[VG_G], nb = 0;
 #pragma unroll
             for (int c = 0; c < VG_G; ++c) {
@@ -218,39 +242,38 @@ __device__ __forceinline__ float vg_class_pass(const VgArgs &a, int lo, int hi, 
                     jc[c] = jc[0];
                 }
             }
-            float f[VG_G][4], uu[VG_G], dqv[VG_G];
+            vg_f4 cf[VG_G];
+            float uu[VG_G], dqv[VG_G];
 #pragma unroll
             for (int c = 0; c < VG_G; ++c) {
                 const int j = jc[c];
                 const float A = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ga), j));
                 const float B = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(gb), j));
-                const int tj = __builtin_amdgcn_readlane(ti, j);
-                const float *Fr = a.prep + (int64_t)tj * a.row;
+                const char *Fr = (const char *)(a.taps + (int64_t)(b0 + j) * a.trow);
                 const float ar = A * r, br = B * r;
+                /* u >= 1e-30 keeps s + th >= 1e-15 > 0 (no second guard) */
                 const float u1 = fmaxf(ar + t2, 1.0e-30f), u2 = fmaxf(br + t2, 1.0e-30f);
                 const float i1 = __builtin_amdgcn_rsqf(u1), i2 = __builtin_amdgcn_rsqf(u2);
-                const float w1 = __builtin_amdgcn_rcpf(fmaxf(__builtin_fmaf(u1, i1, th), 1.0e-30f));
-                const float w2 = __builtin_amdgcn_rcpf(fmaxf(__builtin_fmaf(u2, i2, th), 1.0e-30f));
+                const float w1 = __builtin_amdgcn_rcpf(__builtin_fmaf(u1, i1, th));
+                const float w2 = __builtin_amdgcn_rcpf(__builtin_fmaf(u2, i2, th));
                 const float d = fminf(__builtin_fmaf(ar, w1, br * w2) * a.rdt, dmax);
                 const float Df = floorf(d);
-                const int T = t + (int)Df;
                 uu[c] = d - Df;
                 dqv[c] = __builtin_fmaf(A, i1, B * i2);
-                f[c][0] = Fr[min((unsigned)(T - 1), nsu)];
-                f[c][1] = Fr[min((unsigned)T, nsu)];
-                f[c][2] = Fr[min((unsigned)(T + 1), nsu)];
-                f[c][3] = Fr[min((unsigned)(T + 2), nsu)];
+                /* taps T-1..T+2 at a 32-bit byte offset from the trace's
+                 * (scalar) row base */
+                cf[c] = *(const vg_f4 *)(Fr + (uint32_t)(t + (int)Df) * 4u);
             }
 #pragma unroll
             for (int c = 0; c < VG_G; ++c) {
                 if (c < nb) {
                     if (FWD) {
-                        const float y = cr_eval<false>(f[c], uu[c]) - cmp;
+                        const float y = cr_eval<false>(cf[c], uu[c]) - cmp;
                         const float s = acc + y;
                         cmp = (s - acc) - y;
                         acc = s;
                     } else {
-                        acc = __builtin_fmaf(cr_eval<true>(f[c], uu[c]), dqv[c], acc);
+                        acc = __builtin_fmaf(cr_eval<true>(cf[c], uu[c]), dqv[c], acc);
                     }
                 }
             }
@@ -330,7 +353,7 @@ extern "C" int ktm_vgrad_destroy(ktm_vgrad *p)
     if (!p) return KTM_OK;
     (void)hipSetDevice(p->device);
     void *bufs[] = {p->d_colx, p->d_v, p->d_gv, p->d_part, p->d_G1, p->d_flag,
-                    p->d_order, p->d_ranges};
+                    p->d_order, p->d_ranges, p->d_sgeom, p->d_taps};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free(p->h_part);
@@ -408,6 +431,7 @@ extern "C" int ktm_vgrad_bind(ktm_vgrad *p, const float *prepared, int64_t row,
         return KTM_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
     VG_TRY(hipSetDevice(p->device));
+    p->bound = 0;
     ktm_trace_geom *hg = (ktm_trace_geom *)malloc(sizeof(ktm_trace_geom) * (size_t)ntr);
     vg_key *keys = (vg_key *)malloc(sizeof(vg_key) * (size_t)ntr);
     int32_t *ranges = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)p->nxp * p->nh);
@@ -467,6 +491,20 @@ extern "C" int ktm_vgrad_bind(ktm_vgrad *p, const float *prepared, int64_t row,
             }
             p->order_cap = n;
         }
+        const int64_t trow = (int64_t)p->ns + 6;
+        if (n > p->taps_cap) {
+            (void)hipFree(p->d_sgeom);
+            (void)hipFree(p->d_taps);
+            p->d_sgeom = NULL;
+            p->d_taps = NULL;
+            p->taps_cap = 0;
+            if (hipMalloc(&p->d_sgeom, sizeof(ktm_trace_geom) * (size_t)n) != hipSuccess ||
+                hipMalloc(&p->d_taps, sizeof(float) * (size_t)n * (size_t)trow) != hipSuccess) {
+                rc = KTM_ERR_NOMEM;
+                goto done;
+            }
+            p->taps_cap = n;
+        }
         if ((n > 0 && hipMemcpy(p->d_order, order, sizeof(int32_t) * (size_t)n,
                                 hipMemcpyHostToDevice) != hipSuccess) ||
             hipMemcpy(p->d_ranges, ranges, sizeof(int32_t) * 2 * (size_t)p->nxp * p->nh,
@@ -474,9 +512,15 @@ extern "C" int ktm_vgrad_bind(ktm_vgrad *p, const float *prepared, int64_t row,
             rc = KTM_ERR_DEVICE;
             goto done;
         }
-        p->prep = prepared;
-        p->row = row;
-        p->geom = geom;
+        if (n > 0) {
+            const dim3 grid((unsigned)((trow + 255) / 256), (unsigned)(n < 65535 ? n : 65535));
+            hipLaunchKernelGGL(k_vg_tables, grid, dim3(256), 0, st, prepared, row, geom,
+                               p->d_order, n, p->ns, trow, p->d_taps, p->d_sgeom);
+        }
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+            rc = KTM_ERR_DEVICE;
+            goto done;
+        }
         p->ntr = ntr;
         p->pairs = pairs;
         p->bound = 1;
@@ -511,8 +555,8 @@ extern "C" int ktm_vgrad_eval(ktm_vgrad *p, const float *V, double *J, double *d
     int rc = ktm_vgrad_velocity(p, V, p->d_v, stream);
     if (rc != KTM_OK) return rc;
     VgArgs a;
-    a.prep = p->prep; a.row = p->row; a.geom = p->geom;
-    a.order = p->d_order; a.ranges = p->d_ranges;
+    a.taps = p->d_taps; a.trow = (int64_t)p->ns + 6; a.sgeom = p->d_sgeom;
+    a.ranges = p->d_ranges;
     a.v = p->d_v; a.colx = p->d_colx;
     a.coly = p->tb.coly[0]; a.ap_x2thr = p->tb.ap_x2thr;
     a.dt = p->dt; a.rdt = 1.0f / p->dt;

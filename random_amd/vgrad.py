@@ -10,9 +10,10 @@ writes one common-image gather of nh offset classes per column and
     J(V) = sum_{X,t} sum_{h<nh-1} (I[X][h+1][t] - I[X][h][t])^2
 
 measures how far the gathers are from flat.  ``VelocityGradient`` evaluates
-J and dJ/dV on the GPU: one kernel owns (column, time chunk), accumulates the
-chunk of every offset class in LDS and recomputes the traveltimes for the
-backward pass instead of storing them.  The reference has no such operator
+J and dJ/dV on the GPU: one workgroup owns (column, time chunk), streams the
+offset classes through registers and recomputes the traveltimes for the
+backward pass instead of storing them.  Binding copies the traces into a
+table the plan holds, binned by (offset class, CMP) and zero-padded.  The reference has no such operator
 (its nearest-node velocity, Kirchhoff.c:1357-1358,1376, and floor(dsr/dt),
 1384, have zero derivative almost everywhere).
 """
