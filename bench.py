@@ -64,7 +64,8 @@ def parse():
                     help="traces of the FP64-oracle error check on the 50x50x1500 sub-volume")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy 3x (HBM counter calibration) and exit")
-    ap.add_argument("--leg", choices=("vgrad", "adjoint", "poststack", "eom", "valu", "ingest"),
+    ap.add_argument("--leg", choices=("vgrad", "adjoint", "poststack", "eom", "valu", "ingest",
+                                      "job"),
                     help="only run this extra leg, print its JSON and exit (for profiling)")
     return ap.parse_args()
 
@@ -497,6 +498,20 @@ def ingest_leg(sv, plan, traces, block, dev, reps=5):
     return out
 
 
+def job_only_leg(dev, n=200_000):
+    """The job leg alone on the first n traces of the 3-D survey."""
+    import numpy as np
+
+    from random_amd import surveys, synth
+    from random_amd.ktm import Plan
+    sv = surveys.area3d()
+    sv.coords = np.ascontiguousarray(sv.coords[:n])
+    plan = Plan(sv.cfg, sv.ns, sv.dt, sv.vel)
+    traces, _ = synth.traces_gpu(plan, sv, device=dev)
+    del plan
+    return job_leg(sv, traces, dev, ntr=n)
+
+
 def ingest_only_leg(dev, n=50_000):
     """The ingest leg alone on the first block of the 3-D survey."""
     import numpy as np
@@ -811,6 +826,75 @@ def host_leg(sv, traces, ntr=100_000):
             "host_bytes_out": img_bytes}
 
 
+def job_leg(sv, traces, dev, ntr=200_000, job=0):
+    """kirchhoff(job) end to end, the reference's entry point: the first ntr
+    traces of the 3-D survey written as an IBM SEG-Y file (encoded on the
+    device), main()'s setup (ktm_globals_init: the file counted, the
+    velocity model uploaded; not timed), then one kirchhoff(job) call -- the
+    blocks read from the file, copied, decoded and migrated, the post-stack
+    filter, and image<job>.su encoded and written -- against the device time
+    of its migration launches (ktm_last_stats).  The blocks' input overlaps
+    the previous block's migration, so what the wall time adds to the
+    kernels is the first block's input and the output."""
+    import ctypes as C
+    import os
+    import shutil
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    from random_amd import _lib
+    from random_amd._lib import SEGY_IBM
+    n, ns = min(ntr, traces.shape[0]), sv.ns
+    rec = 240 + 4 * ns
+    hdr = np.zeros((n, 240), np.uint8)
+    hdr[:, 72:88] = sv.coords[:n].astype(">i4").view(np.uint8).reshape(n, 16)
+    hdr[:, 114:116] = np.array([ns], dtype=">u2").view(np.uint8)
+    hdr[:, 116:118] = np.array([round(sv.dt * 1e6)], dtype=">u2").view(np.uint8)
+    d_hdr = torch.from_numpy(hdr).to(dev)
+    raw = torch.empty(n * rec, dtype=torch.uint8, device=dev)
+    _lib.check(_lib.lib().ktm_segy_encode(traces[:n].data_ptr(), n, ns, SEGY_IBM, d_hdr.data_ptr(),
+                                          raw.data_ptr(), torch.cuda.current_stream().cuda_stream),
+               "ktm_segy_encode")
+    host = raw.cpu().numpy()
+    del raw, d_hdr
+    d = tempfile.mkdtemp(prefix="ktm_job_")
+    path = os.path.join(d, "input.sgy")
+    img_dir = os.path.join(d, "image_dir")
+    L = _lib.lib()
+    try:
+        with open(path, "wb") as f:
+            f.write(bytes(3600))
+            f.write(host.tobytes())
+        del host
+        c = sv.cfg.to_c()
+        vel = np.ascontiguousarray(sv.vel, np.float32)
+        _lib.check(L.ktm_globals_init(C.byref(c), path.encode(), None,
+                                      vel.ctypes.data_as(C.c_void_p), vel.shape[2],
+                                      np.float32(sv.dt), img_dir.encode()), "ktm_globals_init")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L.kirchhoff(job)
+        wall = time.perf_counter() - t0
+        _lib.check(L.ktm_last_status(), "kirchhoff")
+        st = _lib.KtmStats()
+        _lib.check(L.ktm_last_stats(C.byref(st)), "ktm_last_stats")
+        out_bytes = os.path.getsize(os.path.join(img_dir, f"image{job}.su"))
+    finally:
+        L.ktm_globals_free()
+        shutil.rmtree(d, ignore_errors=True)
+    blocks = -(-n // sv.cfg.ntblock)
+    log(f"job: kirchhoff({job}) on {n} traces ({blocks} blocks) {wall:.2f} s, migration kernels "
+        f"{st.migrate_ms / 1e3:.2f} s")
+    return {"traces": n, "blocks": int(blocks), "input_file_bytes": 3600 + n * rec,
+            "image_file_bytes": int(out_bytes), "contributions": int(st.contributions),
+            "wall_s": round(wall, 3), "migration_kernels_s": round(st.migrate_ms / 1e3, 3),
+            "kernel_share_of_wall": round(st.migrate_ms / 1e3 / wall, 4),
+            "G_contributions_per_s_end_to_end": round(st.contributions / wall / 1e9, 1),
+            "note": "page-cache reads of a just-written file; image written to the page cache"}
+
+
 def eom_leg(dev, reps=3, cpu_stride=10):
     """Equivalent-offset migration (EOM.c csp()) of the 40-shot line:
     4800 traces x 1001 samples into 800 CSPs x 160 offset bins, plus the post
@@ -862,7 +946,7 @@ def main():
     if args.leg:
         import torch
         fn = {"vgrad": vgrad_leg, "adjoint": adjoint_leg, "poststack": poststack_leg,
-              "eom": eom_leg, "ingest": ingest_only_leg,
+              "eom": eom_leg, "ingest": ingest_only_leg, "job": job_only_leg,
               "valu": lambda d: valu_probe(d, waves_per_simd=(1, 2, 4, 8))}[args.leg]
         print(json.dumps(fn(torch.device("cuda:0"))), flush=True)
         return
@@ -1014,6 +1098,7 @@ def main():
     if not args.profile and not args.no_extra:
         result["ingest"] = ingest_leg(sv, plan, traces, args.block, dev)
         result["host_path"] = host_leg(sv, traces)
+        result["job"] = job_leg(sv, traces, dev)
         result["velocity_gradient"] = vgrad_leg(dev)
         result["adjoint_lsm"] = adjoint_leg(dev)
         result["poststack"] = poststack_leg(dev, plan)

@@ -478,6 +478,11 @@ int  ktm_globals_init(const ktm_config *cfg, const char *input_data,
 void kirchhoff(int job_id);
 /* Status of the last kirchhoff() call (the reference exits instead). */
 int  ktm_last_status(void);
+/* Counts and times of the last kirchhoff() call: its contributions, the
+ * device time of its migration launches and its host wall time, file
+ * input, post-stack and image<job>.su written included (no counterpart in
+ * the reference, which prints per-block timings instead). */
+int  ktm_last_stats(ktm_stats *out);
 void ktm_globals_free(void);
 
 /* combine_all_files() of Kirchhoff.c:821-874 (EOM.c:543-590 is the same):

@@ -138,6 +138,7 @@ _PROTOS = {
     "ktm_globals_init": (_i32, [C.POINTER(KtmConfig), C.c_char_p, C.c_char_p, _vp, _i32, _f32, C.c_char_p]),
     "kirchhoff": (None, [_i32]),
     "ktm_last_status": (_i32, []),
+    "ktm_last_stats": (_i32, [C.POINTER(KtmStats)]),
     "ktm_globals_free": (None, []),
     "ktm_device_sync": (_i32, []),
     "ktm_synth_traces": (_i32, [_vp, _vp, _i64, _vp, _i32, _f32, _f32, _f32, _f32, _vp, _vp]),

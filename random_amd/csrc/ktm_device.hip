@@ -2458,6 +2458,77 @@ extern "C" int ktm_device_sync(void)
     return hipDeviceSynchronize() == hipSuccess ? KTM_OK : KTM_ERR_DEVICE;
 }
 
+/* ---- host pipeline runtime (ktm_internal.h) ---- */
+extern "C" int ktm_rt_pinned_alloc(void **p, size_t bytes)
+{
+    if (!p) return KTM_ERR_ARG;
+    *p = nullptr;
+    return hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault) == hipSuccess ? KTM_OK
+                                                                                  : KTM_ERR_NOMEM;
+}
+extern "C" void ktm_rt_pinned_free(void *p)
+{
+    if (p) (void)hipHostFree(p);
+}
+extern "C" int ktm_rt_stream_create(void **s)
+{
+    if (!s) return KTM_ERR_ARG;
+    *s = nullptr;
+    hipStream_t h = nullptr;
+    if (hipStreamCreate(&h) != hipSuccess) return KTM_ERR_DEVICE;
+    *s = h;
+    return KTM_OK;
+}
+extern "C" void ktm_rt_stream_destroy(void *s)
+{
+    if (s) (void)hipStreamDestroy((hipStream_t)s);
+}
+extern "C" int ktm_rt_stream_sync(void *s)
+{
+    return hipStreamSynchronize((hipStream_t)s) == hipSuccess ? KTM_OK : KTM_ERR_DEVICE;
+}
+extern "C" int ktm_rt_stream_wait(void *s, void *e)
+{
+    return hipStreamWaitEvent((hipStream_t)s, (hipEvent_t)e, 0) == hipSuccess ? KTM_OK
+                                                                            : KTM_ERR_DEVICE;
+}
+extern "C" int ktm_rt_event_create(void **e)
+{
+    if (!e) return KTM_ERR_ARG;
+    *e = nullptr;
+    hipEvent_t h = nullptr;
+    if (hipEventCreate(&h) != hipSuccess) return KTM_ERR_DEVICE;
+    *e = h;
+    return KTM_OK;
+}
+extern "C" void ktm_rt_event_destroy(void *e)
+{
+    if (e) (void)hipEventDestroy((hipEvent_t)e);
+}
+extern "C" int ktm_rt_event_record(void *e, void *s)
+{
+    return hipEventRecord((hipEvent_t)e, (hipStream_t)s) == hipSuccess ? KTM_OK : KTM_ERR_DEVICE;
+}
+extern "C" int ktm_rt_event_sync(void *e)
+{
+    return hipEventSynchronize((hipEvent_t)e) == hipSuccess ? KTM_OK : KTM_ERR_DEVICE;
+}
+extern "C" int ktm_rt_event_elapsed_ms(void *e0, void *e1, float *ms)
+{
+    return hipEventElapsedTime(ms, (hipEvent_t)e0, (hipEvent_t)e1) == hipSuccess ? KTM_OK
+                                                                               : KTM_ERR_DEVICE;
+}
+extern "C" int ktm_rt_copy_h2d_async(void *dst, const void *src, size_t bytes, void *s)
+{
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)s) == hipSuccess
+               ? KTM_OK : KTM_ERR_DEVICE;
+}
+extern "C" int ktm_rt_copy_d2h_async(void *dst, const void *src, size_t bytes, void *s)
+{
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)s) == hipSuccess
+               ? KTM_OK : KTM_ERR_DEVICE;
+}
+
 static int pixel_tables(const ktm_plan *cp, hipStream_t hs)
 {
     ktm_plan *p = const_cast<ktm_plan *>(cp);

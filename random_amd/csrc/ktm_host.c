@@ -277,33 +277,145 @@ static double now_ms(void)
 }
 
 typedef struct dev_bufs {
-    float *traces, *prepared, *km, *fold, *image;
-    int32_t *coords;
-    ktm_trace_geom *geom;
+    float *km, *fold, *image;
     uint64_t *counter;
-    void *raw;
 } dev_bufs;
 
 static void free_bufs(dev_bufs *b)
 {
-    ktm_device_free(b->traces); ktm_device_free(b->prepared); ktm_device_free(b->km);
-    ktm_device_free(b->fold); ktm_device_free(b->image); ktm_device_free(b->coords);
-    ktm_device_free(b->geom); ktm_device_free(b->counter); ktm_device_free(b->raw);
+    ktm_device_free(b->km); ktm_device_free(b->fold); ktm_device_free(b->image);
+    ktm_device_free(b->counter);
     memset(b, 0, sizeof *b);
 }
 
-/* count trace records (240-byte header + ns words each) starting at trace
- * `first` (0-based), as stored in the file */
-static int read_records(FILE *f, int64_t first, int64_t count, int64_t rec, uint8_t *buf)
+/* ------------------------------------------------------------------------ */
+/* host -> device staging                                                    */
+/* ------------------------------------------------------------------------ */
+/* Two pinned chunks: the host fills one (fread from the file, or memcpy from
+ * the caller's arrays) while the DMA out of the other is in flight; each
+ * chunk's event marks its copy done before the chunk is refilled.  For raw
+ * SEG-Y records a chunk holds whole records, so the coordinate words of
+ * each record are checked as it arrives. */
+#define KTM_STAGE_BYTES ((size_t)64 << 20)
+
+typedef struct stage_ring {
+    uint8_t *buf[2];
+    void *done[2];
+    int used[2];
+    size_t cap;
+    int next;
+} stage_ring;
+
+static void ring_free(stage_ring *r)
 {
-    if (fseeko(f, SEGY_FILE_HDR + first * rec, SEEK_SET) != 0) return KTM_ERR_IO;
-    return fread(buf, (size_t)rec, (size_t)count, f) == (size_t)count ? KTM_OK : KTM_ERR_IO;
+    for (int k = 0; k < 2; ++k) {
+        if (r->used[k]) (void)ktm_rt_event_sync(r->done[k]);
+        ktm_rt_event_destroy(r->done[k]);
+        ktm_rt_pinned_free(r->buf[k]);
+    }
+    memset(r, 0, sizeof *r);
+}
+
+static int ring_init(stage_ring *r, size_t cap)
+{
+    memset(r, 0, sizeof *r);
+    r->cap = cap;
+    int st = KTM_OK;
+    for (int k = 0; k < 2 && st == KTM_OK; ++k) {
+        st = ktm_rt_pinned_alloc((void **)&r->buf[k], cap);
+        if (st == KTM_OK) st = ktm_rt_event_create(&r->done[k]);
+    }
+    if (st != KTM_OK) ring_free(r);
+    return st;
+}
+
+/* bytes into dst (device) on stream s, from src or, when f != NULL, read
+ * from f at its position; rec > 0: the bytes are rec-byte SEG-Y trace
+ * records (a chunk holds whole records), and with check their source/
+ * receiver words 18-21 (Kirchhoff.c:1297,1302) must not be negative
+ * (htonl() - ox on such a word is undefined in the reference) */
+static int ring_put(stage_ring *r, void *dst, const void *src, FILE *f, size_t bytes, size_t rec,
+                    int check, void *s)
+{
+    const size_t step = rec ? r->cap / rec * rec : r->cap;
+    for (size_t off = 0; off < bytes; off += step) {
+        const size_t len = bytes - off < step ? bytes - off : step;
+        const int k = r->next;
+        r->next ^= 1;
+        if (r->used[k]) {
+            const int st = ktm_rt_event_sync(r->done[k]);
+            if (st != KTM_OK) return st;
+        }
+        uint8_t *b = r->buf[k];
+        if (f) {
+            if (fread(b, 1, len, f) != len) return KTM_ERR_IO;
+        } else {
+            memcpy(b, (const uint8_t *)src + off, len);
+        }
+        for (size_t o = 0; check && rec && o < len; o += rec)
+            for (int c = 0; c < 4; ++c)
+                if (b[o + 72 + 4 * c] & 0x80u) return KTM_ERR_ARG;
+        int st = ktm_rt_copy_h2d_async((uint8_t *)dst + off, b, len, s);
+        if (st == KTM_OK) st = ktm_rt_event_record(r->done[k], s);
+        if (st != KTM_OK) return st;
+        r->used[k] = 1;
+    }
+    return KTM_OK;
+}
+
+/* bytes of src (device) written to f: each chunk is copied into one pinned
+ * buffer while the host writes the other (stream s) */
+static int ring_write(stage_ring *r, FILE *f, const void *src, size_t bytes, void *s)
+{
+    const size_t nch = (bytes + r->cap - 1) / r->cap;
+    int st = KTM_OK;
+    for (size_t c = 0; c <= nch && st == KTM_OK; ++c) {
+        /* issue chunk c, then write chunk c - 1 (whose buffer's copy is the
+         * older event) */
+        if (c < nch) {
+            const int k = (int)(c & 1);
+            const size_t off = c * r->cap, len = bytes - off < r->cap ? bytes - off : r->cap;
+            if (r->used[k]) st = ktm_rt_event_sync(r->done[k]);
+            if (st == KTM_OK) st = ktm_rt_copy_d2h_async(r->buf[k], (const uint8_t *)src + off, len, s);
+            if (st == KTM_OK) st = ktm_rt_event_record(r->done[k], s);
+            if (st == KTM_OK) r->used[k] = 1;
+        }
+        if (c > 0 && st == KTM_OK) {
+            const int k = (int)((c - 1) & 1);
+            const size_t off = (c - 1) * r->cap, len = bytes - off < r->cap ? bytes - off : r->cap;
+            st = ktm_rt_event_sync(r->done[k]);
+            if (st == KTM_OK && fwrite(r->buf[k], 1, len, f) != len) st = KTM_ERR_IO;
+        }
+    }
+    return st;
+}
+
+/* the file `name` holding bytes of src (device), through a staging ring */
+static int write_device_file(const char *name, const void *src, size_t bytes)
+{
+    FILE *f = fopen(name, "wb");
+    if (!f) return KTM_ERR_IO;
+    stage_ring ring;
+    void *s = NULL;
+    int st = ring_init(&ring, KTM_STAGE_BYTES);
+    if (st == KTM_OK) st = ktm_rt_stream_create(&s);
+    if (st == KTM_OK && bytes) st = ring_write(&ring, f, src, bytes, s);
+    if (s && ktm_rt_stream_sync(s) != KTM_OK && st == KTM_OK) st = KTM_ERR_DEVICE;
+    ring_free(&ring);
+    ktm_rt_stream_destroy(s);
+    if (fclose(f) != 0 && st == KTM_OK) st = KTM_ERR_IO;
+    return st;
 }
 
 /* One pass over all traces in blocks of ntblock (Kirchhoff.c:1276-1408),
  * for every job (job < 0) or one job.  Traces come from the host arrays
  * (traces/coords) or, when path != NULL, from the SEG-Y file: each block of
- * raw records is read as stored and decoded on the device (ktm_ingest). */
+ * raw records is read as stored and decoded on the device (ktm_ingest).
+ * Two device slots and two streams overlap the input of block i + 1 (host
+ * read, copy, decode and prep on the input stream) with the migration of
+ * block i (migration stream); blocks are migrated in order, so the image is
+ * the sequential one bit for bit.  mig_ms sums the migration launches
+ * (events on the migration stream). */
 static int run_survey(const ktm_plan *plan, const ktm_config *cfg, const char *path,
                       const float *traces, const int32_t *coords, int64_t ntr, int job,
                       dev_bufs *b, uint64_t *contrib, double *mig_ms)
@@ -314,59 +426,111 @@ static int run_survey(const ktm_plan *plan, const ktm_config *cfg, const char *p
     const int64_t row = ktm_plan_trace_row(plan);
     const int64_t nb = ntr < blk ? ntr : blk;
     const int64_t rec = SEGY_TRACE_HDR + 4 * (int64_t)s.ns;
-    int st = KTM_OK;
-    uint8_t *hraw = NULL;
+    const int64_t nblocks = ntr > 0 ? (ntr + blk - 1) / blk : 0;
+    struct {
+        void *raw;
+        float *tr, *prepared;
+        int32_t *co;
+        ktm_trace_geom *geom;
+        void *ready, *freed;               /* events: input done, migration done */
+    } slot[2];
+    memset(slot, 0, sizeof slot);
+    void *s_in = NULL, *s_mig = NULL;
+    void **tev = NULL;                     /* [2 * nblocks] migration start/end */
+    stage_ring ring;
+    memset(&ring, 0, sizeof ring);
     FILE *f = NULL;
-    if (path) {
-        hraw = (uint8_t *)malloc((size_t)(nb * rec));
-        if (!hraw) st = KTM_ERR_NOMEM;
-        if (st == KTM_OK && !(f = fopen(path, "rb"))) st = KTM_ERR_IO;
-        if (st == KTM_OK) st = ktm_device_alloc(&b->raw, (size_t)(nb * rec));
-    } else {
-        st = ktm_device_alloc((void **)&b->traces, sizeof(float) * (size_t)(nb * s.ns));
-        if (st == KTM_OK) st = ktm_device_alloc((void **)&b->coords, sizeof(int32_t) * (size_t)(4 * nb));
+    int st = KTM_OK;
+    if (path && !(f = fopen(path, "rb"))) st = KTM_ERR_IO;
+    if (st == KTM_OK)
+        st = ring_init(&ring, path && (size_t)rec > KTM_STAGE_BYTES ? (size_t)rec : KTM_STAGE_BYTES);
+    if (st == KTM_OK) st = ktm_rt_stream_create(&s_in);
+    if (st == KTM_OK) st = ktm_rt_stream_create(&s_mig);
+    if (st == KTM_OK && !(tev = (void **)calloc((size_t)(2 * nblocks + 1), sizeof(void *))))
+        st = KTM_ERR_NOMEM;
+    for (int64_t e = 0; e < 2 * nblocks && st == KTM_OK; ++e) st = ktm_rt_event_create(&tev[e]);
+    for (int k = 0; k < 2 && k < nblocks && st == KTM_OK; ++k) {
+        if (path) {
+            st = ktm_device_alloc(&slot[k].raw, (size_t)(nb * rec));
+        } else {
+            st = ktm_device_alloc((void **)&slot[k].tr, sizeof(float) * (size_t)(nb * s.ns));
+            if (st == KTM_OK)
+                st = ktm_device_alloc((void **)&slot[k].co, sizeof(int32_t) * (size_t)(4 * nb));
+        }
+        if (st == KTM_OK)
+            st = ktm_device_alloc((void **)&slot[k].prepared, sizeof(float) * (size_t)(nb * row));
+        if (st == KTM_OK)
+            st = ktm_device_alloc((void **)&slot[k].geom, sizeof(ktm_trace_geom) * (size_t)nb);
+        if (st == KTM_OK) st = ktm_rt_event_create(&slot[k].ready);
+        if (st == KTM_OK) st = ktm_rt_event_create(&slot[k].freed);
     }
-    if (st == KTM_OK) st = ktm_device_alloc((void **)&b->prepared, sizeof(float) * (size_t)(nb * row));
-    if (st == KTM_OK) st = ktm_device_alloc((void **)&b->geom, sizeof(ktm_trace_geom) * (size_t)nb);
     if (st == KTM_OK) st = ktm_device_alloc((void **)&b->counter, sizeof(uint64_t));
     uint64_t zero = 0;
     if (st == KTM_OK) st = ktm_copy_to_device(b->counter, &zero, sizeof zero);
-    double t_mig = 0.0;
-    for (int64_t i0 = 0; i0 < ntr && st == KTM_OK; i0 += blk) {
+    for (int64_t bi = 0; bi < nblocks && st == KTM_OK; ++bi) {
+        const int64_t i0 = bi * blk;
         const int64_t n = ntr - i0 < blk ? ntr - i0 : blk;
+        const int k = (int)(bi & 1);
+        /* the slot's buffers are free once the migration of block bi - 2 ends */
+        if (bi >= 2) st = ktm_rt_stream_wait(s_in, slot[k].freed);
         if (path) {
-            st = read_records(f, i0, n, rec, hraw);
-            /* header words 18-21 (Kirchhoff.c:1297,1302): htonl() - ox on a
-             * negative word is undefined in the reference */
-            for (int64_t k = 0; k < n && st == KTM_OK; ++k)
-                for (int c = 0; c < 4; ++c)
-                    if (hraw[k * rec + 72 + 4 * c] & 0x80u) st = KTM_ERR_ARG;
-            if (st == KTM_OK) st = ktm_copy_to_device(b->raw, hraw, (size_t)(n * rec));
+            if (st == KTM_OK && fseeko(f, SEGY_FILE_HDR + i0 * rec, SEEK_SET) != 0) st = KTM_ERR_IO;
             if (st == KTM_OK)
-                st = ktm_ingest(plan, b->raw, n, KTM_SEGY_IBM, KTM_PREP_REFERENCE, KTM_TRACE_F32,
-                                b->prepared, b->geom, NULL, NULL);
+                st = ring_put(&ring, slot[k].raw, NULL, f, (size_t)(n * rec), (size_t)rec, 1, s_in);
+            if (st == KTM_OK)
+                st = ktm_ingest(plan, slot[k].raw, n, KTM_SEGY_IBM, KTM_PREP_REFERENCE,
+                                KTM_TRACE_F32, slot[k].prepared, slot[k].geom, NULL, s_in);
         } else {
             const float *src_tr = traces + i0 * s.ns;
             const int32_t *src_co = coords + 4 * i0;
-            for (int64_t k = 0; k < 4 * n && st == KTM_OK; ++k)
-                if (src_co[k] < 0) st = KTM_ERR_ARG; /* htonl() - ox on a negative word is undefined */
-            if (st == KTM_OK) st = ktm_copy_to_device(b->traces, src_tr, sizeof(float) * (size_t)(n * s.ns));
-            if (st == KTM_OK) st = ktm_copy_to_device(b->coords, src_co, sizeof(int32_t) * (size_t)(4 * n));
+            for (int64_t q = 0; q < 4 * n && st == KTM_OK; ++q)
+                if (src_co[q] < 0) st = KTM_ERR_ARG; /* htonl() - ox on a negative word is undefined */
             if (st == KTM_OK)
-                st = ktm_prep(plan, b->traces, b->coords, n, KTM_PREP_REFERENCE, KTM_TRACE_F32,
-                              b->prepared, b->geom, NULL);
+                st = ring_put(&ring, slot[k].tr, src_tr, NULL, sizeof(float) * (size_t)(n * s.ns), 0,
+                              0, s_in);
+            if (st == KTM_OK)
+                st = ring_put(&ring, slot[k].co, src_co, NULL, sizeof(int32_t) * (size_t)(4 * n), 0,
+                              0, s_in);
+            if (st == KTM_OK)
+                st = ktm_prep(plan, slot[k].tr, slot[k].co, n, KTM_PREP_REFERENCE, KTM_TRACE_F32,
+                              slot[k].prepared, slot[k].geom, s_in);
         }
-        const double t0 = now_ms();
+        if (st == KTM_OK) st = ktm_rt_event_record(slot[k].ready, s_in);
+        if (st == KTM_OK) st = ktm_rt_stream_wait(s_mig, slot[k].ready);
+        if (st == KTM_OK) st = ktm_rt_event_record(tev[2 * bi], s_mig);
         if (st == KTM_OK)
-            st = ktm_migrate_block(plan, b->prepared, KTM_TRACE_F32, KTM_PREP_REFERENCE, b->geom, n,
-                                   KTM_ACCUM_PLAIN, job, b->km, b->fold, NULL, b->counter, NULL);
-        if (st == KTM_OK) st = ktm_device_sync();
-        t_mig += now_ms() - t0;
+            st = ktm_migrate_block(plan, slot[k].prepared, KTM_TRACE_F32, KTM_PREP_REFERENCE,
+                                   slot[k].geom, n, KTM_ACCUM_PLAIN, job, b->km, b->fold, NULL,
+                                   b->counter, s_mig);
+        if (st == KTM_OK) st = ktm_rt_event_record(tev[2 * bi + 1], s_mig);
+        if (st == KTM_OK) st = ktm_rt_event_record(slot[k].freed, s_mig);
+    }
+    /* drain both streams before anything is freed, on success or not */
+    if (s_in && ktm_rt_stream_sync(s_in) != KTM_OK && st == KTM_OK) st = KTM_ERR_DEVICE;
+    if (s_mig && ktm_rt_stream_sync(s_mig) != KTM_OK && st == KTM_OK) st = KTM_ERR_DEVICE;
+    double t_mig = 0.0;
+    for (int64_t bi = 0; bi < nblocks && st == KTM_OK; ++bi) {
+        float ms = 0.0f;
+        st = ktm_rt_event_elapsed_ms(tev[2 * bi], tev[2 * bi + 1], &ms);
+        t_mig += ms;
     }
     if (st == KTM_OK) st = ktm_copy_to_host(contrib, b->counter, sizeof(uint64_t));
     *mig_ms = t_mig;
+    ring_free(&ring);
+    for (int k = 0; k < 2; ++k) {
+        ktm_device_free(slot[k].raw);
+        ktm_device_free(slot[k].tr);
+        ktm_device_free(slot[k].co);
+        ktm_device_free(slot[k].prepared);
+        ktm_device_free(slot[k].geom);
+        ktm_rt_event_destroy(slot[k].ready);
+        ktm_rt_event_destroy(slot[k].freed);
+    }
+    for (int64_t e = 0; tev && e < 2 * nblocks; ++e) ktm_rt_event_destroy(tev[e]);
+    free(tev);
+    ktm_rt_stream_destroy(s_in);
+    ktm_rt_stream_destroy(s_mig);
     if (f) fclose(f);
-    free(hraw);
     return st;
 }
 
@@ -425,9 +589,17 @@ static struct {
     int ns, nt;
     float dt;
     ktm_plan *plan;
+    ktm_stats stats;
 } G;
 
 int ktm_last_status(void) { return G.status; }
+
+int ktm_last_stats(ktm_stats *out)
+{
+    if (!out) return KTM_ERR_ARG;
+    *out = G.stats;
+    return KTM_OK;
+}
 
 void ktm_globals_free(void)
 {
@@ -485,7 +657,9 @@ int ktm_globals_init(const ktm_config *cfg, const char *input_data, const char *
 
 static int kirchhoff_job(int job_id)
 {
+    memset(&G.stats, 0, sizeof G.stats);
     if (!G.ready) return KTM_ERR_ARG;
+    const double t_start = now_ms();
     ktm_setup s;
     ktm_plan_setup(G.plan, &s);
     if (job_id < 0 || job_id >= s.no_job) return KTM_ERR_ARG;
@@ -500,6 +674,8 @@ static int kirchhoff_job(int job_id)
     double ms = 0.0;
     if (st == KTM_OK)
         st = run_survey(G.plan, &G.cfg, G.input_data, NULL, NULL, G.nt, job_id, &b, &contrib, &ms);
+    G.stats.contributions = contrib;
+    G.stats.migrate_ms = ms;
     if (st == KTM_OK) st = ktm_poststack(G.plan, b.km, b.fold, b.image, NULL);
     /* image<job>.su: per (xc, yc, ih) a 240-byte header and ns big-endian
      * IEEE floats, in the order of Kirchhoff.c:1440-1598, encoded on the
@@ -507,21 +683,16 @@ static int kirchhoff_job(int job_id)
     const size_t nrec = (size_t)s.nx * s.ny * s.nh;
     const size_t bytes = nrec * (SEGY_TRACE_HDR + 4 * (size_t)s.ns);
     void *drec = NULL;
-    uint8_t *hrec = NULL;
     if (st == KTM_OK) st = ktm_device_alloc(&drec, bytes);
     if (st == KTM_OK) st = ktm_image_records(G.plan, b.image, job_id, drec, NULL);
-    if (st == KTM_OK && !(hrec = (uint8_t *)malloc(bytes))) st = KTM_ERR_NOMEM;
-    if (st == KTM_OK) st = ktm_copy_to_host(hrec, drec, bytes);
-    ktm_device_free(drec);
-    free_bufs(&b);
-    if (st != KTM_OK) { free(hrec); return st; }
     char name[4200];
     snprintf(name, sizeof name, "%s/image%d.su", G.image_dir, job_id);
-    FILE *f = fopen(name, "wb");
-    if (!f) { free(hrec); return KTM_ERR_IO; }
-    if (fwrite(hrec, 1, bytes, f) != bytes) st = KTM_ERR_IO;
-    free(hrec);
-    if (fclose(f) != 0 && st == KTM_OK) st = KTM_ERR_IO;
+    /* the records stream from the device into the file (a stream ordered
+     * after the records kernel on the null stream) */
+    if (st == KTM_OK) st = write_device_file(name, drec, bytes);
+    ktm_device_free(drec);
+    free_bufs(&b);
+    G.stats.total_ms = now_ms() - t_start;
     return st;
 }
 
@@ -618,38 +789,74 @@ static int eom_csp_job(int job_id)
     const int64_t nb = E.nt < blk ? E.nt : blk;
     const int64_t rec = SEGY_TRACE_HDR + 4 * (int64_t)s.ns;
     const size_t nrec_max = (size_t)s.nx * s.ny * s.nh;
-    float *d_csp = NULL, *d_fold = NULL, *d_out = NULL, *d_tr = NULL;
-    int32_t *d_co = NULL;
-    void *d_raw = NULL, *d_rec = NULL;
-    uint8_t *hbuf = NULL;
+    const int64_t nblocks = E.nt > 0 ? (E.nt + blk - 1) / blk : 0;
+    float *d_csp = NULL, *d_fold = NULL, *d_out = NULL;
+    void *d_rec = NULL;
+    struct {
+        void *raw;
+        float *tr;
+        int32_t *co;
+        void *ready, *freed;               /* events: input done, binning done */
+    } slot[2];
+    memset(slot, 0, sizeof slot);
+    void *s_in = NULL, *s_bin = NULL;
+    stage_ring ring;
+    memset(&ring, 0, sizeof ring);
     FILE *f = NULL;
     /* csp_data = 0, fold = 1 for the job's slab (EOM.c:1192-1211) */
     int st = ktm_device_alloc((void **)&d_csp, sizeof(float) * ng);
     if (st == KTM_OK) st = ktm_device_alloc((void **)&d_fold, sizeof(float) * ng);
     if (st == KTM_OK) st = ktm_eom_job_reset(E.plan, d_csp, d_fold, NULL);
     /* the trace loop in blocks of ntblock (EOM.c:1221-1316): raw records
-     * read as stored, IBM-decoded and binned on the device */
-    if (st == KTM_OK && nb > 0) {
-        if (!(hbuf = (uint8_t *)malloc((size_t)(nb * rec)))) st = KTM_ERR_NOMEM;
-        if (st == KTM_OK && !(f = fopen(E.input_data, "rb"))) st = KTM_ERR_IO;
-        if (st == KTM_OK) st = ktm_device_alloc(&d_raw, (size_t)(nb * rec));
-        if (st == KTM_OK) st = ktm_device_alloc((void **)&d_tr, sizeof(float) * (size_t)(nb * s.ns));
-        if (st == KTM_OK) st = ktm_device_alloc((void **)&d_co, sizeof(int32_t) * (size_t)(4 * nb));
-    }
-    for (int64_t i0 = 0; i0 < E.nt && st == KTM_OK; i0 += blk) {
-        const int64_t n = E.nt - i0 < blk ? E.nt - i0 : blk;
-        st = read_records(f, i0, n, rec, hbuf);
-        if (st == KTM_OK) st = ktm_copy_to_device(d_raw, hbuf, (size_t)(n * rec));
-        if (st == KTM_OK) st = ktm_segy_decode(d_raw, n, s.ns, KTM_SEGY_IBM, d_tr, d_co, NULL);
+     * read as stored, IBM-decoded and binned on the device; the input of
+     * block i + 1 (read, copy, decode on one stream) overlaps the binning of
+     * block i (another stream), the blocks binned in order */
+    if (st == KTM_OK && nblocks > 0) {
+        if (!(f = fopen(E.input_data, "rb"))) st = KTM_ERR_IO;
         if (st == KTM_OK)
-            st = ktm_eom_job_block(E.plan, d_tr, d_co, n, job_id, d_csp, d_fold, NULL, NULL);
+            st = ring_init(&ring, (size_t)rec > KTM_STAGE_BYTES ? (size_t)rec : KTM_STAGE_BYTES);
+        if (st == KTM_OK) st = ktm_rt_stream_create(&s_in);
+        if (st == KTM_OK) st = ktm_rt_stream_create(&s_bin);
+        for (int k = 0; k < 2 && k < nblocks && st == KTM_OK; ++k) {
+            st = ktm_device_alloc(&slot[k].raw, (size_t)(nb * rec));
+            if (st == KTM_OK)
+                st = ktm_device_alloc((void **)&slot[k].tr, sizeof(float) * (size_t)(nb * s.ns));
+            if (st == KTM_OK)
+                st = ktm_device_alloc((void **)&slot[k].co, sizeof(int32_t) * (size_t)(4 * nb));
+            if (st == KTM_OK) st = ktm_rt_event_create(&slot[k].ready);
+            if (st == KTM_OK) st = ktm_rt_event_create(&slot[k].freed);
+        }
     }
+    for (int64_t bi = 0; bi < nblocks && st == KTM_OK; ++bi) {
+        const int64_t i0 = bi * blk;
+        const int64_t n = E.nt - i0 < blk ? E.nt - i0 : blk;
+        const int k = (int)(bi & 1);
+        if (bi >= 2) st = ktm_rt_stream_wait(s_in, slot[k].freed);
+        if (st == KTM_OK && fseeko(f, SEGY_FILE_HDR + i0 * rec, SEEK_SET) != 0) st = KTM_ERR_IO;
+        if (st == KTM_OK)
+            st = ring_put(&ring, slot[k].raw, NULL, f, (size_t)(n * rec), (size_t)rec, 0, s_in);
+        if (st == KTM_OK)
+            st = ktm_segy_decode(slot[k].raw, n, s.ns, KTM_SEGY_IBM, slot[k].tr, slot[k].co, s_in);
+        if (st == KTM_OK) st = ktm_rt_event_record(slot[k].ready, s_in);
+        if (st == KTM_OK) st = ktm_rt_stream_wait(s_bin, slot[k].ready);
+        if (st == KTM_OK)
+            st = ktm_eom_job_block(E.plan, slot[k].tr, slot[k].co, n, job_id, d_csp, d_fold, NULL,
+                                   s_bin);
+        if (st == KTM_OK) st = ktm_rt_event_record(slot[k].freed, s_bin);
+    }
+    if (s_in && ktm_rt_stream_sync(s_in) != KTM_OK && st == KTM_OK) st = KTM_ERR_DEVICE;
+    if (s_bin && ktm_rt_stream_sync(s_bin) != KTM_OK && st == KTM_OK) st = KTM_ERR_DEVICE;
+    ring_free(&ring);
+    for (int k = 0; k < 2; ++k) {
+        ktm_device_free(slot[k].raw);
+        ktm_device_free(slot[k].tr);
+        ktm_device_free(slot[k].co);
+        ktm_rt_event_destroy(slot[k].ready);
+        ktm_rt_event_destroy(slot[k].freed);
+    }
+    ktm_rt_stream_destroy(s_in);
+    ktm_rt_stream_destroy(s_bin);
     if (f) fclose(f);
-    free(hbuf);
-    hbuf = NULL;
-    ktm_device_free(d_raw);
-    ktm_device_free(d_tr);
-    ktm_device_free(d_co);
     /* scale, low-pass and sqrt-rho filter (EOM.c:1386-1413), then the
      * records of image<job>.su (EOM.c:1346-1537), all on the device */
     if (st == KTM_OK) st = ktm_device_alloc((void **)&d_out, sizeof(float) * ng);
@@ -659,17 +866,13 @@ static int eom_csp_job(int job_id)
     int64_t nrec = 0;
     if (st == KTM_OK) st = ktm_device_alloc(&d_rec, nrec_max * (size_t)rec);
     if (st == KTM_OK) st = ktm_eom_job_records(E.plan, d_out, job_id, d_rec, &nrec, NULL);
-    const size_t bytes = (size_t)nrec * (size_t)rec;
-    if (st == KTM_OK && bytes && !(hbuf = (uint8_t *)malloc(bytes))) st = KTM_ERR_NOMEM;
-    if (st == KTM_OK && bytes) st = ktm_copy_to_host(hbuf, d_rec, bytes);
-    ktm_device_free(d_out);
-    ktm_device_free(d_rec);
     if (st == KTM_OK) {
         char name[4200];
         snprintf(name, sizeof name, "%s/image%d.su", E.image_dir, job_id);
-        st = write_whole(name, hbuf, bytes);
+        st = write_device_file(name, d_rec, (size_t)nrec * (size_t)rec);
     }
-    free(hbuf);
+    ktm_device_free(d_out);
+    ktm_device_free(d_rec);
     return st;
 }
 

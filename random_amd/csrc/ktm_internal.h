@@ -39,6 +39,23 @@ void ktm_tables_free(ktm_tables *tb);
  * -1 where the reference would read below the model. */
 int  ktm_vel_index(const ktm_config *cfg, int col, int axis);
 
+/* Host pipeline runtime for the C host code (ktm_device.hip): pinned host
+ * memory, streams at the default priority that are ordered with work on the
+ * null stream (hipStreamCreate), and timing events.  Handles are opaque. */
+int  ktm_rt_pinned_alloc(void **p, size_t bytes);
+void ktm_rt_pinned_free(void *p);
+int  ktm_rt_stream_create(void **s);
+void ktm_rt_stream_destroy(void *s);
+int  ktm_rt_stream_sync(void *s);
+int  ktm_rt_stream_wait(void *s, void *e);          /* s waits for e's work */
+int  ktm_rt_event_create(void **e);
+void ktm_rt_event_destroy(void *e);
+int  ktm_rt_event_record(void *e, void *s);
+int  ktm_rt_event_sync(void *e);                     /* the host waits for e */
+int  ktm_rt_event_elapsed_ms(void *e0, void *e1, float *ms);
+int  ktm_rt_copy_h2d_async(void *dst, const void *src, size_t bytes, void *s);
+int  ktm_rt_copy_d2h_async(void *dst, const void *src, size_t bytes, void *s);
+
 /* Shared by the C host code: big-endian helpers. */
 static inline void ktm_put_be32(uint8_t *o, uint32_t u)
 {

@@ -45,14 +45,19 @@ def test_kirchhoff_job_files(cuda, tmp_path):
     img_dir = str(tmp_path / "image_dir")
     _lib.check(L.ktm_globals_init(C.byref(c), path.encode(), None, vp(sv.vel), sv.ns,
                                   np.float32(sv.dt), img_dir.encode()), "ktm_globals_init")
-    km_o, fold_o, _ = oracle.migrate(sv.cfg, sv.ns, sv.dt, tr_ibm, sv.coords, sv.vel)
+    km_o, fold_o, n_o = oracle.migrate(sv.cfg, sv.ns, sv.dt, tr_ibm, sv.coords, sv.vel)
     tm, aa = oracle.alias_filter(sv.cfg, sv.ns, sv.dt)
     img_o = oracle.poststack(km_o, fold_o, sv.dt, aa)
     g = oracle.grid(sv.cfg)
+    n_jobs = 0
     try:
         for job in range(g.no_job):
             L.kirchhoff(job)
             assert L.ktm_last_status() == 0
+            st = _lib.KtmStats()
+            _lib.check(L.ktm_last_stats(C.byref(st)), "ktm_last_stats")
+            assert 0.0 < st.migrate_ms <= st.total_ms
+            n_jobs += st.contributions
             data = open(os.path.join(img_dir, f"image{job}.su"), "rb").read()
             rec = 240 + 4 * sv.ns
             assert len(data) == g.nx * g.ny * sv.cfg.nh * rec
@@ -63,6 +68,8 @@ def test_kirchhoff_job_files(cuda, tmp_path):
                 got = np.frombuffer(r[240:], ">f4").astype(np.float32)
                 ref = img_o[job * g.nx + k, 0, 0]
                 assert np.linalg.norm(got - ref) <= 1e-5 * max(np.linalg.norm(ref), 1e-30)
+        # every (trace, column) pair in aperture belongs to one job's slab
+        assert n_jobs == n_o
         fh = open(os.path.join(img_dir, "binary_header.bin"), "rb").read()
         assert fh == oracle.segy_file_header(sv.cfg, sv.ns, sv.dt)[3200:]
         # main()'s combine step (Kirchhoff.c:355, 821-874): the script, and the
