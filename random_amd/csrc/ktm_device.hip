@@ -1466,6 +1466,11 @@ __global__ void k_demig_keys(const ktm_trace_geom *__restrict__ geom, int64_t nt
  * segment, not k*64 + l: far from the CMP, T(t) grows slowly at early
  * times, and 64 consecutive samples land on a few T, whose atomics then
  * serialise; samples ~13 apart land on distinct T in distinct banks.
+ * A sample whose fast-form q lies within the form's error bound of an
+ * integer needs the reference arithmetic: each lane defers its first such
+ * sample of a column and the wave resolves them after the column's traces,
+ * every lane's together (the integer sums do not depend on the order of the
+ * additions); a lane's further ones in the column take it inline.
  * FASTIDX: the index floor(min(q, ns + 1/2)) of the fast form alone, the
  * transpose of k_migrate's FASTIDX variant (floor(min(q, ns)) there: both
  * are ns for q >= ns, so they index the same samples). */
@@ -1555,6 +1560,10 @@ k_demigrate(const DemigArgs a)
             mq[k] = 0;
         }
         int cur = -1;                      /* offset class whose samples mq holds */
+        /* one ambiguous sample per lane deferred to the end of the column:
+         * its slot | trace << 8 (-1: none) and its fixed-point value */
+        int qkj = -1;
+        long long qv = 0;
         while (pm) {
             const int j = __builtin_ctzll(pm);
             pm &= pm - 1ull;
@@ -1577,11 +1586,33 @@ k_demigrate(const DemigArgs a)
                 const float qc = fast_clamp(fast_qs(dIS2, dIR2, rs[k], tq[k]), nsh);
                 int T = fast_T(qc);
                 if (!FASTIDX && fast_ambiguous(qc, KTM_FAST_EPS_S)) {   /* never past the trace */
-                    const int t = slot_t(k);
-                    const float v2 = a.pv2[pixc + t];
-                    T = exact_T(dIS2, dIR2, a.t2[t], v2, 1.0f / v2, a.dt, a.rdt, ns);
+                    if (qkj < 0) {
+                        /* deferred: the integer sums do not depend on the
+                         * order of their additions */
+                        qkj = k | (j << 8);
+                        qv = mq[k];
+                        T = ns;
+                    } else {
+                        const int t = slot_t(k);
+                        const float v2 = a.pv2[pixc + t];
+                        T = exact_T(dIS2, dIR2, a.t2[t], v2, 1.0f / v2, a.dt, a.rdt, ns);
+                    }
                 }
                 if (T < ns) atomicAdd(&gl[T], (unsigned long long)mq[k]);
+            }
+        }
+        if (!FASTIDX && __ballot(qkj >= 0)) {
+            /* the deferred samples of the column, every lane's together:
+             * one reference-arithmetic pass instead of one per ambiguous
+             * lane and slot (the trace's distances from its lane, all lanes
+             * active for the shuffle) */
+            const int jq = qkj >= 0 ? qkj >> 8 : 0;
+            const float dIS2 = __shfl(g_is, jq), dIR2 = __shfl(g_ir, jq);
+            if (qkj >= 0) {
+                const int t = slot_t(qkj & 0xff);
+                const float v2 = a.pv2[pixc + t];
+                const int T = exact_T(dIS2, dIR2, a.t2[t], v2, 1.0f / v2, a.dt, a.rdt, ns);
+                if (T < ns) atomicAdd(&sg[jq * ldg + T], (unsigned long long)qv);
             }
         }
     }
