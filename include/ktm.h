@@ -16,7 +16,11 @@
  * of ktm_demigrate) are reused by
  * every call without stream ordering: all calls on one plan must be issued
  * on a single stream (or the caller orders them), or a later call can
- * overwrite a workspace an earlier call's kernels still read.
+ * overwrite a workspace an earlier call's kernels still read.  ktm_prep,
+ * ktm_ingest and ktm_geometry use none of these workspaces, nor does
+ * ktm_migrate_block on FP32 traces with plain sums: the block loop of
+ * ktm_migrate and kirchhoff() runs the input of one block beside the
+ * migration of the previous one on a second stream for that reason.
  */
 #ifndef KTM_H
 #define KTM_H
