@@ -210,7 +210,8 @@ __global__ void k_vg_tables(const float *__restrict__ prep, int64_t row,
  * [0, ns + 2] for every lane (m >= 0; inactive lanes have r = 0). */
 template <bool FWD>
 __device__ __forceinline__ float vg_class_pass(const VgArgs &a, int lo, int hi, float IX,
-                                               float r, float th, float t2, int t, int lane)
+                                               float r, float th, float t2, int t, int lane,
+                                               float2 *sAB)
 {
     const float dmax = (float)(a.ns + 2 - t);   /* q clamped to ns + 2: a zero interval */
     float acc = 0.0f, cmp = 0.0f;
@@ -227,6 +228,11 @@ __device__ __forceinline__ float vg_class_pass(const VgArgs &a, int lo, int hi, 
             gb = bx * bx + by * by;
             pass = cx * cx + cy * cy < a.ap_x2thr;
         }
+        /* the squared distances of the wave's 64 traces in its LDS row: a
+         * passing trace's pair is one broadcast read below, not two
+         * v_readlane per contribution (the wave's own LDS accesses are
+         * ordered, so the row is rewritten only after the reads above it) */
+        sAB[lane] = make_float2(ga, gb);
         uint64_t m = __ballot(pass);
         while (m) {
             /* up to VG_G passing traces at a time, in order; slots past the
@@ -247,8 +253,8 @@ __device__ __forceinline__ float vg_class_pass(const VgArgs &a, int lo, int hi, 
 #pragma unroll
             for (int c = 0; c < VG_G; ++c) {
                 const int j = jc[c];
-                const float A = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ga), j));
-                const float B = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(gb), j));
+                const float2 ab = sAB[j];
+                const float A = ab.x, B = ab.y;
                 const char *Fr = (const char *)(a.taps + (int64_t)(b0 + j) * a.trow);
                 const float ar = A * r, br = B * r;
                 /* u >= 1e-30 keeps s + th >= 1e-15 > 0 (no second guard) */
@@ -285,6 +291,7 @@ __device__ __forceinline__ float vg_class_pass(const VgArgs &a, int lo, int hi, 
 __global__ void __launch_bounds__(VG_NT) k_vgrad(const VgArgs a)
 {
     __shared__ double sRed[VG_NT / VG_WAVE];
+    __shared__ float2 sABw[VG_NT / VG_WAVE][VG_WAVE];   /* per wave: the gate lanes' distances */
     const int tid = threadIdx.x, lane = tid & (VG_WAVE - 1), wv = tid / VG_WAVE;
     /* XCD-aware order: consecutive workgroup ids go to different XCDs; each
      * XCD takes runs of a.run consecutive (column, chunk) tiles (neighbouring
@@ -315,12 +322,12 @@ __global__ void __launch_bounds__(VG_NT) k_vgrad(const VgArgs a)
      * so no LDS bounds the occupancy. */
     double jl = 0.0;
     float dprev = 0.0f, g = 0.0f;
-    float Ih = vg_class_pass<true>(a, rg[0], rg[1], IX, r, th, t2, t, lane);
+    float Ih = vg_class_pass<true>(a, rg[0], rg[1], IX, r, th, t2, t, lane, sABw[wv]);
     if (!act) Ih = 0.0f;
     for (int h = 0; h < a.nh; ++h) {
         float In = 0.0f, dn = 0.0f;
         if (h + 1 < a.nh) {
-            In = vg_class_pass<true>(a, rg[2 * h + 2], rg[2 * h + 3], IX, r, th, t2, t, lane);
+            In = vg_class_pass<true>(a, rg[2 * h + 2], rg[2 * h + 3], IX, r, th, t2, t, lane, sABw[wv]);
             if (!act) In = 0.0f;
             dn = In - Ih;
         }
@@ -328,7 +335,7 @@ __global__ void __launch_bounds__(VG_NT) k_vgrad(const VgArgs a)
         if (a.cig && act) a.cig[((int64_t)X * a.nh + h) * a.ns + t] = Ih;
         if (a.gv) {
             const float rh = 2.0f * (dprev - dn);
-            const float S = vg_class_pass<false>(a, rg[2 * h], rg[2 * h + 1], IX, r, th, t2, t, lane);
+            const float S = vg_class_pass<false>(a, rg[2 * h], rg[2 * h + 1], IX, r, th, t2, t, lane, sABw[wv]);
             g = __builtin_fmaf(rh, S, g);
         }
         dprev = dn;
