@@ -257,8 +257,9 @@ __device__ __forceinline__ float vg_class_pass(const VgArgs &a, int lo, int hi, 
                 const float A = ab.x, B = ab.y;
                 const char *Fr = (const char *)(a.taps + (int64_t)(b0 + j) * a.trow);
                 const float ar = A * r, br = B * r;
-                /* u >= 1e-30 keeps s + th >= 1e-15 > 0 (no second guard) */
-                const float u1 = fmaxf(ar + t2, 1.0e-30f), u2 = fmaxf(br + t2, 1.0e-30f);
+                /* t2 carries +1e-30 (see k_vgrad): u >= 1e-30 keeps s + th
+                 * >= 1e-15 > 0 without a guard per contribution */
+                const float u1 = ar + t2, u2 = br + t2;
                 const float i1 = __builtin_amdgcn_rsqf(u1), i2 = __builtin_amdgcn_rsqf(u2);
                 const float w1 = __builtin_amdgcn_rcpf(__builtin_fmaf(u1, i1, th));
                 const float w2 = __builtin_amdgcn_rcpf(__builtin_fmaf(u2, i2, th));
@@ -313,6 +314,14 @@ __global__ void __launch_bounds__(VG_NT) k_vgrad(const VgArgs a)
         t2 = th * th;
         kfac = -(r / v) / a.dt;     /* dq/dv = kfac (a/s1 + b/s2) */
     }
+    /* + 1e-30 replaces a guard u = max(a r + t2, 1e-30) per contribution
+     * (u = 0, at t = 0 with a source or receiver on the column, would give
+     * s + th = 0): for t >= 1 (t2 >= 4e-6) t2 is unchanged, and at t = 0
+     * u = a r + 1e-30 equals the guarded value for a r = 0 and for
+     * a r >= 3.4e-23 (1e-30 below half its ulp) -- every case but a source
+     * or receiver within ~6e-8 m of the column and not on it, where u moves
+     * by at most 1e-30 */
+    t2 = t2 + 1.0e-30f;
     const int32_t *rg = a.ranges + (int64_t)X * a.nh * 2;
 
     /* The offset classes stream in order: the lane holds I[h-1..h+1] of its
