@@ -1598,7 +1598,10 @@ k_demigrate(const DemigArgs a)
                         T = exact_T(dIS2, dIR2, a.t2[t], v2, 1.0f / v2, a.dt, a.rdt, ns);
                     }
                 }
-                if (T < ns) atomicAdd(&gl[T], (unsigned long long)mq[k]);
+                /* T = ns (past the trace, or deferred) adds into the pad
+                 * g[ns], which the transpose below never reads: no compare
+                 * and no branch per slot */
+                atomicAdd(&gl[T], (unsigned long long)mq[k]);
             }
         }
         if (!FASTIDX && __ballot(qkj >= 0)) {
