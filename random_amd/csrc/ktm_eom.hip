@@ -313,7 +313,10 @@ struct EomArgs {
 
 __global__ void __launch_bounds__(EOM_NT) k_eom_accum(const EomArgs a)
 {
-    __shared__ float sX2[EOM_NT], sH2[EOM_NT];
+    /* per passing trace: 4 x2 h2 (exact in double) and RN(x2 + h2), the
+     * trace-only terms of EOM.c:1285-1286 */
+    __shared__ double sXH4[EOM_NT];
+    __shared__ float sXpH[EOM_NT];
     __shared__ int sTa[EOM_NT], sIdx[EOM_NT], sCnt[EOM_NT / EOM_WAVE];
     const int tid = threadIdx.x, lane = tid & (EOM_WAVE - 1), wv = tid / EOM_WAVE;
     /* XCD-aware order: consecutive workgroup ids go to different XCDs; each
@@ -372,23 +375,26 @@ __global__ void __launch_bounds__(EOM_NT) k_eom_accum(const EomArgs a)
             total += sCnt[q];
         }
         if (pass) {
-            sX2[base + pre] = x2;
-            sH2[base + pre] = h2;
+            sXH4[base + pre] = 4.0 * (double)x2 * (double)h2;
+            sXpH[base + pre] = x2 + h2;
             sTa[base + pre] = ta;
             sIdx[base + pre] = tid;
         }
         __syncthreads();
         for (int j = 0; j < total; ++j) {
-            const float x2j = sX2[j], h2j = sH2[j];
             const int taj = sTa[j];
             const bool live = tin && t >= taj;
             nev += __popcll(__ballot(live));
             if (!live) continue;
             /* EOM.c:1285-1306 with the reference's promotions */
-            const float cross = (float)div_cr(4.0 * (double)x2j * (double)h2j, (double)den0, rden0);
-            const float arg = x2j + h2j - cross;
+            const float cross = (float)div_cr(sXH4[j], (double)den0, rden0);
+            const float arg = sXpH[j] - cross;
             if (!(arg >= 0.0f)) continue;                 /* NaN offset: no contribution */
-            const float eq = (float)sqrt_cr((double)arg);
+            /* (float)sqrt((double)arg) of the reference: the double square
+             * root of a float, rounded to float, is the correctly rounded
+             * float square root (double rounding is innocuous for sqrt when
+             * 53 >= 2 x 24 + 2), which __builtin_sqrtf is without fast-math */
+            const float eq = __builtin_sqrtf(arg);
             const double qo = div_cr((double)(eq - (float)a.oh), a.dh, rdh);
             const double fl = floor(qo);
             if (!(fl >= 0.0 && fl < (double)(a.nh - 1))) continue;
