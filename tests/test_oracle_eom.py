@@ -279,3 +279,19 @@ def test_eom_file_header():
     assert struct.unpack(">4H", b[20:28]) == (1500, 1500, 5, 1)
     assert struct.unpack(">2H", b[28:32]) == (1, 1)
     assert not any(b[:16]) and not any(b[32:])
+
+
+def test_float_sqrt_through_double_is_correctly_rounded():
+    """EOM.c:1286 takes sqrt(x2 + h2 - crossterm2) in double of a float
+    argument and stores it in a float; the GPU binning kernel computes the
+    correctly rounded float square root instead.  The two agree because
+    double rounding is innocuous for sqrt when the intermediate precision
+    (53 bits) is at least 2 x 24 + 2: checked here on every 97th
+    non-negative float, denormals and +inf included (numpy's float32 sqrt
+    is the correctly rounded one)."""
+    bits = np.arange(0, 0x7F800001, 97, dtype=np.uint32)
+    bits = np.concatenate([bits, np.array([1, 0x007FFFFF, 0x00800000, 0x7F7FFFFF, 0x7F800000],
+                                          np.uint32)])
+    x = bits.view(np.float32)
+    via_double = np.sqrt(x.astype(np.float64)).astype(np.float32)
+    assert np.array_equal(via_double.view(np.uint32), np.sqrt(x).view(np.uint32))
