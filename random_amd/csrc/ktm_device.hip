@@ -529,11 +529,12 @@ struct MigArgs {
     int ns, nh, nxp, nyp, nx, ny, jobs_y, nvel_xl, nsm;
     int tiles_x, ntiles, tx_cols, ty_cols, run;
     int rx0, ry0, rx1, ry1;          /* migrated region [rx0,rx1) x [ry0,ry1) */
-    /* Kahan variants: the block's traces in parts of part_len (a multiple of
-     * the 512-trace chunk), part = blockIdx.z; part 0 adds into km/fold/comp,
-     * part p > 0 writes its sums into parts + (p-1) * 3 * part_n, region
-     * layout [X-rx0][Y-ry0][h][t] */
-    int64_t part_len, part_n;
+    /* Kahan variants: the block's 512-trace chunks dealt round-robin to
+     * nparts parts (chunk c to part c mod nparts), part = blockIdx.z, so that
+     * every part spans the whole survey and its workgroups carry like loads;
+     * part 0 adds into km/fold/comp, part p > 0 writes its sums into
+     * parts + (p-1) * 3 * part_n, region layout [X-rx0][Y-ry0][h][t] */
+    int64_t nparts, part_n;
     float *parts;
 };
 
@@ -811,9 +812,10 @@ k_migrate(const MigArgs a)
 
     uint32_t npass = 0;
     const int n4 = ldl / 4;
-    const int64_t cbeg = KAHAN ? part * a.part_len : 0;
-    const int64_t cend = KAHAN ? min(a.ntr, cbeg + a.part_len) : a.ntr;
-    for (int64_t c0 = cbeg; c0 < cend; c0 += NT) {
+    const int64_t cbeg = KAHAN ? (int64_t)part * NT : 0;
+    const int64_t cstep = KAHAN ? a.nparts * NT : NT;
+    const int64_t cend = a.ntr;
+    for (int64_t c0 = cbeg; c0 < cend; c0 += cstep) {
         /* ---- select the traces of this chunk that can reach the tile, in order ---- */
         const int64_t i = c0 + tid;
         bool pred = false;
@@ -2411,17 +2413,15 @@ static int migrate_block(const ktm_plan *p, const void *prepared, int dtype, int
     const int nthreads = 8 * KTM_WAVE;
     /* compensated sums of a small region: the traces in parts */
     int nparts = 1;
-    a.part_len = ntr;
     a.part_n = ncols_region * s->nh * s->ns;
     a.parts = p->d_parts;
     if (acc_mode == KTM_ACCUM_KAHAN && p->parts_max > 1) {
         nparts = parts_for((int64_t)a.ntiles * s->nh);
         if (nparts > p->parts_max) nparts = p->parts_max;
-        if (nparts > 1) {
-            a.part_len = ((ntr + nparts - 1) / nparts + nthreads - 1) / nthreads * nthreads;
-            nparts = (int)((ntr + a.part_len - 1) / a.part_len);
-        }
+        const int64_t nchunks = (ntr + nthreads - 1) / nthreads;
+        if (nparts > nchunks) nparts = (int)nchunks;     /* no part without a chunk */
     }
+    a.nparts = nparts;
     /* staged rows + geometry, selection list and counts, and: v^2 per sample
      * and the speculation queue (exact index) or the fold row V (fast index) */
     const bool fast = (accum & KTM_INDEX_FAST) != 0;
