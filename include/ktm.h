@@ -202,9 +202,9 @@ int ktm_image_reset(const ktm_plan *plan, float *km, float *fold, float *comp,
  * successive blocks equal one pass over their concatenation (bit for bit
  * with PLAIN; with KAHAN within the compensated bound).  With KAHAN a block
  * whose image region is small (fewer than 10240 workgroups of tiles x
- * offset classes) is split along its traces into up to 32 parts summed
- * separately and merged in FP64 (deterministic: the same call gives the
- * same bits); the partial images live in the plan, allocated by
+ * offset classes) is split along its traces into up to 32 parts (its
+ * 512-trace chunks dealt round-robin) summed separately and merged in FP64
+ * (deterministic: the same call gives the same bits); the partial images live in the plan, allocated by
  * ktm_plan_create for an image that small (at most 1 GiB).  If counter is
  * non-NULL (device uint64) the number of contributions is added to it: ns per
  * (trace, column) pair through the aperture gates of Kirchhoff.c:1338,1354,
